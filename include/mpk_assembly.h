@@ -1,0 +1,207 @@
+/*
+ * mpk_assembly.h -- C ABI of the MI355X element-assembly engine for MagPackage.
+ *
+ * One shared library, libmpk_assembly.so (gfx950 HIP kernels).  Plain
+ * pointers and sizes only.  Every function returns an mpk status code
+ * (MPK_OK == 0); on failure mpk_last_error() returns a thread-local message.
+ *
+ * Pointer spaces:
+ *   - "device" arrays live in HBM of the current HIP device;
+ *   - "host" arrays live in host memory;
+ *   - quadrature tables are always host arrays (they are copied into kernel
+ *     arguments).
+ * Streams: `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * Calls are asynchronous on `stream` unless stated otherwise; device outputs
+ * are valid once the stream has run them.  The library never allocates the
+ * caller's outputs.
+ *
+ * Index conventions follow the reference (MATLAB): node ids, DOF ids and
+ * region tags are 1-based; a DOF of 0 marks an inactive entity, whose
+ * triplets Assembler.m filters out (src/Assembler.m:37).
+ */
+#ifndef MPK_ASSEMBLY_H
+#define MPK_ASSEMBLY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define MPK_OK 0
+#define MPK_ERR_ARGS 1   /* wrong argument count/shape/null pointer ("MagPackage:Args") */
+#define MPK_ERR_INDEX 2  /* node / DOF / tag index outside its range */
+#define MPK_ERR_HIP 3    /* HIP runtime error */
+#define MPK_ERR_LIMIT 4  /* a size limit of the engine (message says which) */
+
+/* ---- precision modes of the CSC assembly --------------------------------- */
+/* FP64, reference operation order, no FMA contraction: bit-identical to the
+ * MEX kernels followed by sparse() summing duplicates in triplet order. */
+#define MPK_PREC_FP64_EXACT 0
+/* FP64 closed forms with FMA (default): <= 1e-12 relative to FP64_EXACT. */
+#define MPK_PREC_FP64 1
+/* Local block entries in FP32, determinant/inverse and CSC accumulation in
+ * FP64: <= 1e-6 relative. */
+#define MPK_PREC_FP32_LOCAL 2
+
+int mpk_abi_version(void);
+const char* mpk_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int mpk_device_count(void);
+
+/* Quadrature tables in the reference's layouts (host memory).
+ * qw[nq]; ref_curl and ref_val are [6 x 3 x nq] column-major (index
+ * i + 6*c + 18*q), as produced by nedelec_tet_p1 (src/shape_functions/
+ * nedelec_tet_p1.m:18-51) for get_quadrature_data (src/utils/
+ * get_quadrature_data.m:12-29); q_pts is [3 x nq] column-major.  nq <= 8. */
+typedef struct mpk_quadrature {
+    int32_t nq;
+    double qw[8];
+    double q_pts[24];
+    double ref_val[144];
+    double ref_curl[144];
+} mpk_quadrature;
+
+/* Packed B-H material tables, the 8 SoA arrays Assembler.packMaterialData
+ * builds (src/Assembler.m:270-304) and assemble_jacobian_kernel_mex reads
+ * (src/mex/MexMaterialUtils.hpp:9-18).  Device arrays. */
+typedef struct mpk_material {
+    const double* linear_nu;    /* [num_tags] */
+    const double* is_nonlinear; /* [num_tags] */
+    const double* max_bsq;      /* [num_tags] */
+    const double* breaks;       /* [num_breaks] */
+    const double* coefs;        /* [num_coefs], per piece c3 c2 c1 c0 */
+    const double* spline_start; /* [num_tags], 0-based offsets into breaks */
+    const double* spline_count; /* [num_tags], number of pieces */
+    const double* coef_start;   /* [num_tags], 0-based offsets into coefs */
+    int64_t num_tags;
+    int64_t num_breaks;
+    int64_t num_coefs;
+} mpk_material;
+
+/* ========================================================================== */
+/* 1. MEX-compatible element kernels: COO triplets, reference layouts          */
+/* ========================================================================== */
+/* All arrays device, doubles, MATLAB layouts: P [3 x nn], T [4 x ne] (1-based),
+ * cell_dofs / signs [6 x ne], per-element coefficients [ne].  Outputs are
+ * 36*ne triplets in the reference order: slot 36*e + 6*c + r holds
+ * (dofs[r], dofs[c], K[r][c]*s[r]*s[c]).  Node ids are checked against nn
+ * (MPK_ERR_INDEX); these calls synchronise `stream` for that check. */
+
+/* Replaces assemble_curl_curl_kernel_mex (src/mex/assemble_curl_curl_kernel_mex.cpp:18-121). */
+int mpk_curl_curl_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                      const double* cell_dofs, const double* signs, const double* nu,
+                      const mpk_quadrature* quad, double* I_out, double* J_out,
+                      double* V_out, void* stream);
+
+/* Replaces assemble_mass_kernel_mex (src/mex/assemble_mass_kernel_mex.cpp:16-127).
+ * coeffs may be NULL or have num_coeffs != ne: sigma = 1 then (:34,66). */
+int mpk_mass_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                 const double* cell_dofs, const double* signs, const double* coeffs,
+                 int64_t num_coeffs, const mpk_quadrature* quad, double* I_out,
+                 double* J_out, double* V_out, void* stream);
+
+/* Replaces assemble_jacobian_kernel_mex (src/mex/assemble_jacobian_kernel_mex.cpp:7-153).
+ * sol_a has num_sol entries (indexed dof-1).  calc_j == 0: I/J/V untouched
+ * (the MEX returns 0x1 arrays), R triplets still written. */
+int mpk_jacobian_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                     const double* cell_dofs, const double* signs, const double* tags,
+                     const double* sol_a, int64_t num_sol, const mpk_quadrature* quad,
+                     const mpk_material* mat, int calc_j, double* I_out, double* J_out,
+                     double* V_out, double* R_idx, double* R_val, void* stream);
+
+/* ========================================================================== */
+/* 2. COO -> CSC with MATLAB sparse() semantics                               */
+/* ========================================================================== */
+/* Replaces `valid = (I>0)&(J>0); sparse(I(valid),J(valid),V(valid),m,n)`
+ * (src/Assembler.m:37-38).  Device arrays.  Duplicates are summed in triplet
+ * order; exact zeros are dropped when drop_zeros != 0 (MATLAB's rule).
+ * colptr_out [n+1] (int64), rowind_out [cap] (0-based int32), vals_out [cap].
+ * *nnz_out receives nnz; if nnz > cap, MPK_ERR_LIMIT is returned and the
+ * outputs are not written (cap = nt always suffices).  Synchronises `stream`. */
+int mpk_sparse_coo_to_csc(const double* I, const double* J, const double* V, int64_t nt,
+                          int64_t m, int64_t n, int drop_zeros, int64_t* colptr_out,
+                          int32_t* rowind_out, double* vals_out, int64_t cap,
+                          int64_t* nnz_out, void* stream);
+
+/* ========================================================================== */
+/* 3. Mesh topology and DOF map (device, engine-native int32 layouts)          */
+/* ========================================================================== */
+/* Replaces Mesh.generateEdges (src/Mesh.m:114-138).  T: [ne x 4] rows of
+ * 1-based node ids (== MATLAB T [4 x ne] column-major).  edges_out: [2 x cap]
+ * (cap >= 6*ne) 1-based sorted unique (min,max) pairs; t2e_out [6 x ne]
+ * 1-based edge ids; sign_out [6 x ne] +1 iff T(a) <= T(b).  Synchronises. */
+int mpk_generate_edges(const int32_t* T, int64_t ne, int64_t nn, int32_t* edges_out,
+                       int32_t* t2e_out, int8_t* sign_out, int64_t* nedge_out,
+                       void* stream);
+
+/* Replaces DofHandler.distributeDofs, Nedelec branch (src/DofHandler.m:72-105).
+ * active_elem [ne] (NULL = all active); entity_map [nedge]; cell_dofs [6 x ne].
+ * Synchronises. */
+int mpk_distribute_dofs(const int32_t* t2e, int64_t ne, int64_t nedge,
+                        const uint8_t* active_elem, int64_t offset, int32_t* entity_map,
+                        int32_t* cell_dofs, int64_t* ndofs_out, void* stream);
+
+/* ========================================================================== */
+/* 4. CSC assembly plan: pattern + incidence lists, built once per mesh/DOF map */
+/* ========================================================================== */
+/* The pattern is sparse()'s structural pattern of the 36*ne triplets with
+ * both DOFs > 0 (no value-dependent dropping; see mpk_csc_drop_zeros).  For
+ * every column j it stores the incident (element, local edge) pairs in
+ * ascending element order, so accumulation follows triplet order. */
+typedef struct mpk_plan mpk_plan;
+
+/* T: [ne x 4] 1-based node ids, cell_dofs: [ne x 6] 1-based DOFs (0 =
+ * inactive), both device int32.  ndofs = NumGlobalDofs.  Synchronises. */
+int mpk_plan_create(const int32_t* T, const int32_t* cell_dofs, int64_t ne, int64_t nn,
+                    int64_t ndofs, void* stream, mpk_plan** plan_out);
+int mpk_plan_destroy(mpk_plan* plan);
+/* nnz, number of columns, elements, incidences and the largest LDS tile. */
+int mpk_plan_info(const mpk_plan* plan, int64_t* nnz, int64_t* ncols, int64_t* ne,
+                  int64_t* n_incidences, int64_t* max_tile_nnz);
+/* Device pointers owned by the plan: colptr [n+1], rowind [nnz] (0-based). */
+int mpk_plan_pattern(const mpk_plan* plan, const int32_t** colptr, const int32_t** rowind);
+
+/* CSC values of K (curl-curl, src/mex/assemble_curl_curl_kernel_mex.cpp:18-121
+ * + sparse()).  P [3 x nn] device doubles, nu [ne] device.  vals [nnz]. */
+int mpk_csc_curl_curl(const mpk_plan* plan, const double* P, const double* nu,
+                      const mpk_quadrature* quad, int precision, double* vals, void* stream);
+
+/* CSC values of the (sigma-)mass (src/mex/assemble_mass_kernel_mex.cpp:16-127
+ * + sparse()).  coeffs [ne] device or NULL (sigma = 1). */
+int mpk_csc_mass(const mpk_plan* plan, const double* P, const double* coeffs,
+                 const mpk_quadrature* quad, int precision, double* vals, void* stream);
+
+/* Complex128 CSC values of K + 1j*omega*M_sigma (src/solver/FrequencySolver.m:
+ * 127-131 composition), fused: real = sum K, imag = omega * sum M.
+ * vals_c128 [2*nnz] interleaved (re, im).  quad_k / quad_m: stiffness and
+ * mass tables (Assembler.m:30-32 and :64-66). */
+int mpk_csc_curl_curl_mass_complex(const mpk_plan* plan, const double* P, const double* nu,
+                                   const double* sigma, double omega,
+                                   const mpk_quadrature* quad_k, const mpk_quadrature* quad_m,
+                                   int precision, double* vals_c128, void* stream);
+
+/* Newton Jacobian J and residual R (src/mex/assemble_jacobian_kernel_mex.cpp
+ * + sparse() of J and of R, src/Assembler.m:199-216).  tags [ne] device int32
+ * (1-based); sol_a [ndofs] device.  j_vals [nnz] (ignored when calc_j == 0);
+ * residual [ndofs]. */
+int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
+                     const double* sol_a, const mpk_material* mat,
+                     const mpk_quadrature* quad, int calc_j, int precision,
+                     double* j_vals, double* residual, void* stream);
+
+/* Emulates sparse()'s value-dependent rule on an assembled structural CSC:
+ * copies the entries that are not exactly zero into (colptr_out [n+1] int64,
+ * rowind_out, vals_out) (capacity nnz) and returns the new nnz.  `complex`
+ * != 0 treats vals as interleaved complex128 (an entry is dropped when both
+ * parts are zero).  Synchronises. */
+int mpk_csc_drop_zeros(const mpk_plan* plan, const double* vals, int complex_vals,
+                       int64_t* colptr_out, int32_t* rowind_out, double* vals_out,
+                       int64_t* nnz_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPK_ASSEMBLY_H */

@@ -1,0 +1,381 @@
+"""Host-side mirror of MagPackage's assembly layer on the MI355X engine.
+
+Same class and method names as the reference (snake_case aliases), same
+argument meaning and the same results (CSC matrices equal to what
+``sparse()`` returns), computed by the HIP kernels of libmpk_assembly.so:
+
+- ``Mesh``          src/Mesh.m  (P, T, RegionTags; ``generate_edges`` :114-138)
+- ``FunctionSpace`` src/FunctionSpace.m
+- ``FemConfig``     src/FemConfig.m:8-17
+- ``DofHandler``    src/DofHandler.m:23-130 (Nedelec spaces)
+- ``Assembler``     src/Assembler.m:26-76, 182-268
+
+Device arrays are ``torch`` tensors on ``cuda:0`` (torch is used for device
+memory and streams only).  Matrices come back as :class:`CscMatrix` with
+device-resident ``colptr`` / ``rowind`` / ``values``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .fem_tables import get_quadrature_data, matlab_flat, nedelec_tet_p1
+from .material import pack_material_data
+
+
+def _dev():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+@dataclass
+class FemConfig:
+    AssemblyChunkSize: int = 2000
+    DefaultQuadratureOrder: int = 2
+    LinearTolerance: float = 1e-6
+    DebugMode: bool = False
+
+    @staticmethod
+    def Default() -> "FemConfig":
+        return FemConfig()
+
+
+class FunctionSpace:
+    def __init__(self, type_: str = "Nedelec", order: int = 1):
+        self.Type = type_
+        self.Order = order
+
+    def toString(self) -> str:
+        return f"{self.Type}_P{self.Order}"
+
+    to_string = toString
+
+
+class Mesh:
+    """Tetrahedral mesh.  ``P``: (Nn, 3) float64, ``T``: (Ne, 4) 1-based ids."""
+
+    def __init__(self, P, T, RegionTags=None):
+        self.P = np.ascontiguousarray(P, dtype=np.float64)
+        self.T = np.ascontiguousarray(T, dtype=np.int32)
+        ne = self.T.shape[0]
+        self.RegionTags = (np.ones(ne, dtype=np.int32) if RegionTags is None
+                           else np.ascontiguousarray(RegionTags, dtype=np.int32))
+        self.NumNodes = int(self.P.shape[0])
+        self.NumElements = int(ne)
+        dev = _dev()
+        self.P_dev = torch.from_numpy(self.P.reshape(-1)).to(dev)
+        self.T_dev = torch.from_numpy(self.T.reshape(-1)).to(dev)
+        self.Tags_dev = torch.from_numpy(self.RegionTags).to(dev)
+        self.Edges = None
+        self.T2E = None
+        self.T2E_Sign = None
+
+    @staticmethod
+    def from_kuhn(km) -> "Mesh":
+        return Mesh(km.P, km.T, km.RegionTags)
+
+    def generateEdges(self) -> None:
+        """Mesh.generateEdges (src/Mesh.m:114-138) on the GPU."""
+        if self.Edges is not None:
+            return
+        ne, nn = self.NumElements, self.NumNodes
+        dev = _dev()
+        edges = torch.empty(max(12 * ne, 2), dtype=torch.int32, device=dev)
+        t2e = torch.empty(max(6 * ne, 1), dtype=torch.int32, device=dev)
+        sign = torch.empty(max(6 * ne, 1), dtype=torch.int8, device=dev)
+        nedge = C.c_int64(0)
+        L.check(L.lib.mpk_generate_edges(L.ptr(self.T_dev), ne, nn, L.ptr(edges), L.ptr(t2e),
+                                         L.ptr(sign), C.byref(nedge), L.stream_ptr()))
+        E = int(nedge.value)
+        self.Edges = edges[: 2 * E].view(E, 2)
+        self.T2E = t2e[: 6 * ne].view(ne, 6)
+        self.T2E_Sign = sign[: 6 * ne].view(ne, 6)
+        self.NumEdges = E
+
+    generate_edges = generateEdges
+
+
+class DofHandler:
+    def __init__(self, mesh: Mesh):
+        self.Mesh = mesh
+        self.NumGlobalDofs = 0
+        self.DofMaps = {}
+        self.SpaceOffsets = {}
+        self.SpaceLocalSizes = {}
+        self.EntityMaps = {}
+
+    def distributeDofs(self, space: FunctionSpace, targetRegionTags=None):
+        """DofHandler.distributeDofs, Nedelec branch (src/DofHandler.m:72-105)."""
+        key = space.toString()
+        if key in self.DofMaps:
+            return self.DofMaps[key], self.SpaceOffsets[key]
+        if space.Type.lower() != "nedelec" or space.Order != 1:
+            raise ValueError(f"Unsupported space: {key} (the engine covers Nedelec P1)")
+        mesh = self.Mesh
+        mesh.generateEdges()
+        ne, E = mesh.NumElements, mesh.NumEdges
+        dev = _dev()
+        active = None
+        if targetRegionTags is not None and len(np.atleast_1d(targetRegionTags)) > 0:
+            act = np.isin(mesh.RegionTags, np.atleast_1d(targetRegionTags)).astype(np.uint8)
+            active = torch.from_numpy(act).to(dev)
+        emap = torch.empty(max(E, 1), dtype=torch.int32, device=dev)
+        cell = torch.empty(max(6 * ne, 1), dtype=torch.int32, device=dev)
+        nd = C.c_int64(0)
+        offset = self.NumGlobalDofs
+        L.check(L.lib.mpk_distribute_dofs(L.ptr(mesh.T2E), ne, E, L.ptr(active), offset,
+                                          L.ptr(emap), L.ptr(cell), C.byref(nd), L.stream_ptr()))
+        cell_dofs = cell[: 6 * ne].view(ne, 6)
+        self.DofMaps[key] = cell_dofs
+        self.SpaceOffsets[key] = offset
+        self.SpaceLocalSizes[key] = int(nd.value)
+        self.EntityMaps[key] = emap[:E]
+        self.NumGlobalDofs += int(nd.value)
+        return cell_dofs, offset
+
+    distribute_dofs = distributeDofs
+
+    def packForKernel(self, space: FunctionSpace) -> dict:
+        key = space.toString()
+        return {"CellDofs": self.DofMaps[key], "Signs": self.Mesh.T2E_Sign}
+
+
+class CscMatrix:
+    """Device CSC matrix (0-based row indices), MATLAB's sparse storage."""
+
+    def __init__(self, shape, colptr, rowind, values):
+        self.shape = tuple(int(s) for s in shape)
+        self.colptr = colptr
+        self.rowind = rowind
+        self.values = values
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowind.numel())
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        v = self.values.detach().cpu().numpy()
+        return sp.csc_matrix((v, self.rowind.cpu().numpy().astype(np.int64),
+                              self.colptr.cpu().numpy().astype(np.int64)), shape=self.shape)
+
+
+class Plan:
+    """Owner of an mpk_plan (pattern + incidence lists for one DOF map)."""
+
+    def __init__(self, mesh: Mesh, cell_dofs: torch.Tensor, ndofs: int):
+        self._p = C.c_void_p(None)
+        L.check(L.lib.mpk_plan_create(L.ptr(mesh.T_dev), L.ptr(cell_dofs.reshape(-1)),
+                                      mesh.NumElements, mesh.NumNodes, ndofs, L.stream_ptr(),
+                                      C.byref(self._p)))
+        nnz, nc, ne, ninc, mt = (C.c_int64() for _ in range(5))
+        L.check(L.lib.mpk_plan_info(self._p, C.byref(nnz), C.byref(nc), C.byref(ne),
+                                    C.byref(ninc), C.byref(mt)))
+        self.nnz, self.n, self.ne = nnz.value, nc.value, ne.value
+        self.n_incidences, self.max_tile_nnz = ninc.value, mt.value
+        cp, ri = C.c_void_p(), C.c_void_p()
+        L.check(L.lib.mpk_plan_pattern(self._p, C.byref(cp), C.byref(ri)))
+        self._colptr_addr, self._rowind_addr = cp.value, ri.value
+
+    @property
+    def handle(self):
+        return self._p
+
+    def pattern(self):
+        """(colptr int64 [n+1], rowind int32 [nnz]) copies on the device."""
+        dev = _dev()
+        colptr = torch.empty(self.n + 1, dtype=torch.int32, device=dev)
+        rowind = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        # copy the plan-owned pattern (device to device, stream-ordered)
+        _memcpy_d2d(colptr.data_ptr(), self._colptr_addr, (self.n + 1) * 4)
+        if self.nnz:
+            _memcpy_d2d(rowind.data_ptr(), self._rowind_addr, self.nnz * 4)
+        return colptr.to(torch.int64), rowind[: self.nnz]
+
+    def drop_zeros(self, values: torch.Tensor, complex_vals: bool) -> CscMatrix:
+        dev = _dev()
+        colptr = torch.empty(self.n + 1, dtype=torch.int64, device=dev)
+        rowind = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        vout = torch.empty_like(values)
+        nnz = C.c_int64(0)
+        L.check(L.lib.mpk_csc_drop_zeros(self._p, L.ptr(values), int(complex_vals), L.ptr(colptr),
+                                         L.ptr(rowind), L.ptr(vout), C.byref(nnz),
+                                         L.stream_ptr()))
+        k = int(nnz.value)
+        return CscMatrix((self.n, self.n), colptr, rowind[:k], vout[:k])
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value:
+            L.lib.mpk_plan_destroy(p)
+            self._p = C.c_void_p(None)
+
+
+def _memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
+    # torch has no raw-pointer copy: wrap both addresses as byte tensors.
+    src_t = _wrap_device_bytes(src, nbytes)
+    dst_t = _wrap_device_bytes(dst, nbytes)
+    dst_t.copy_(src_t)
+
+
+def _wrap_device_bytes(addr: int, nbytes: int) -> torch.Tensor:
+    class _CAI:
+        __cuda_array_interface__ = {"shape": (nbytes,), "typestr": "|u1",
+                                    "data": (addr, False), "version": 3, "strides": None}
+    return torch.as_tensor(_CAI(), device=_dev())
+
+
+def expand_material_map(map_or_array, tags: np.ndarray) -> np.ndarray:
+    """Assembler.expandMaterialMap (src/Assembler.m:240-268)."""
+    ne = len(tags)
+    if isinstance(map_or_array, dict):
+        vec = np.zeros(ne)
+        for k, v in map_or_array.items():
+            kv = float(k) if not isinstance(k, str) else float(k)
+            if np.isscalar(v):
+                vec[tags == kv] = v
+        return vec
+    a = np.asarray(map_or_array, dtype=np.float64)
+    if a.ndim == 0 or a.size == 1:
+        return np.full(ne, float(a.ravel()[0]))
+    if a.ndim == 1 or min(a.shape) == 1:
+        a = a.ravel()
+        if a.size == ne:
+            return a.copy()
+        return a[tags - 1]
+    raise ValueError("Assembler:InvalidInput Invalid material array format.")
+
+
+class Assembler:
+    def __init__(self, mesh: Mesh, dofHandler: DofHandler, config: FemConfig | None = None):
+        self.Mesh = mesh
+        self.DofHandler = dofHandler
+        self.Config = config or FemConfig.Default()
+        self._plans = {}
+
+    # -- helpers ---------------------------------------------------------
+    def plan(self, space: FunctionSpace) -> Plan:
+        key = (space.toString(), self.DofHandler.NumGlobalDofs)
+        if key not in self._plans:
+            cell = self.DofHandler.DofMaps[space.toString()]
+            self._plans[key] = Plan(self.Mesh, cell, self.DofHandler.NumGlobalDofs)
+        return self._plans[key]
+
+    def _elem_vec(self, map_or_array) -> torch.Tensor:
+        v = expand_material_map(map_or_array, self.Mesh.RegionTags)
+        return torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(_dev())
+
+    def stiffness_quadrature(self) -> L.Quadrature:
+        return L.standard_quadrature(max(self.Config.DefaultQuadratureOrder, 2))
+
+    def mass_quadrature(self, weighted: bool) -> L.Quadrature:
+        o = self.Config.DefaultQuadratureOrder
+        return L.standard_quadrature(o if weighted else max(o, 2))
+
+    def _finish(self, plan: Plan, vals: torch.Tensor, complex_vals: bool, drop_zeros: bool):
+        if drop_zeros:
+            m = plan.drop_zeros(vals, complex_vals)
+        else:
+            colptr, rowind = plan.pattern()
+            m = CscMatrix((plan.n, plan.n), colptr, rowind, vals)
+        if complex_vals:
+            m.values = torch.view_as_complex(m.values.view(-1, 2))
+        return m
+
+    # -- assembly (src/Assembler.m) ---------------------------------------
+    def assembleStiffness(self, space, materialMap, precision: str = "fp64",
+                          drop_zeros: bool = True, out: torch.Tensor | None = None):
+        plan = self.plan(space)
+        nu = self._elem_vec(materialMap)
+        vals = out if out is not None else torch.empty(max(plan.nnz, 1), dtype=torch.float64,
+                                                       device=_dev())
+        q = self.stiffness_quadrature()
+        L.check(L.lib.mpk_csc_curl_curl(plan.handle, L.ptr(self.Mesh.P_dev), L.ptr(nu),
+                                        C.byref(q), L.PRECISIONS[precision], L.ptr(vals),
+                                        L.stream_ptr()))
+        return self._finish(plan, vals[: plan.nnz], False, drop_zeros)
+
+    def assembleMass(self, space, precision: str = "fp64", drop_zeros: bool = True):
+        plan = self.plan(space)
+        vals = torch.empty(max(plan.nnz, 1), dtype=torch.float64, device=_dev())
+        q = self.mass_quadrature(weighted=False)
+        L.check(L.lib.mpk_csc_mass(plan.handle, L.ptr(self.Mesh.P_dev), None, C.byref(q),
+                                   L.PRECISIONS[precision], L.ptr(vals), L.stream_ptr()))
+        return self._finish(plan, vals[: plan.nnz], False, drop_zeros)
+
+    def assembleMassWeighted(self, space, coeffMap, precision: str = "fp64",
+                             drop_zeros: bool = True):
+        plan = self.plan(space)
+        coeff = self._elem_vec(coeffMap)
+        vals = torch.empty(max(plan.nnz, 1), dtype=torch.float64, device=_dev())
+        q = self.mass_quadrature(weighted=True)
+        L.check(L.lib.mpk_csc_mass(plan.handle, L.ptr(self.Mesh.P_dev), L.ptr(coeff), C.byref(q),
+                                   L.PRECISIONS[precision], L.ptr(vals), L.stream_ptr()))
+        return self._finish(plan, vals[: plan.nnz], False, drop_zeros)
+
+    def assembleFrequencyOperator(self, space, nuMap, sigmaMap, omega: float,
+                                  precision: str = "fp64", drop_zeros: bool = True):
+        """K + 1j*omega*M_sigma (FrequencySolver.m:127-131 without the eps_reg term)."""
+        plan = self.plan(space)
+        nu = self._elem_vec(nuMap)
+        sg = self._elem_vec(sigmaMap)
+        vals = torch.empty(max(2 * plan.nnz, 2), dtype=torch.float64, device=_dev())
+        qk = self.stiffness_quadrature()
+        qm = self.mass_quadrature(weighted=True)
+        L.check(L.lib.mpk_csc_curl_curl_mass_complex(
+            plan.handle, L.ptr(self.Mesh.P_dev), L.ptr(nu), L.ptr(sg), float(omega), C.byref(qk),
+            C.byref(qm), L.PRECISIONS[precision], L.ptr(vals), L.stream_ptr()))
+        return self._finish(plan, vals[: 2 * plan.nnz], True, drop_zeros)
+
+    def material_tables(self, matLibData):
+        """Device copy of the packed B-H tables + the ctypes struct pointing at them."""
+        pk = pack_material_data(matLibData)
+        dev = _dev()
+        keep = {k: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)
+                for k, v in pk.items()}
+        m = L.Material()
+        m.linear_nu = L.ptr(keep["linNu"]); m.is_nonlinear = L.ptr(keep["isNon"])
+        m.max_bsq = L.ptr(keep["maxB"]); m.breaks = L.ptr(keep["breaks"])
+        m.coefs = L.ptr(keep["coefs"]); m.spline_start = L.ptr(keep["starts"])
+        m.spline_count = L.ptr(keep["counts"]); m.coef_start = L.ptr(keep["coe_starts"])
+        m.num_tags = keep["linNu"].numel()
+        m.num_breaks = keep["breaks"].numel()
+        m.num_coefs = keep["coefs"].numel()
+        return m, keep
+
+    def assembleJacobian(self, space, solutionA, matLibData, calcJ: bool = True,
+                         precision: str = "fp64", drop_zeros: bool = True, tables=None):
+        """[J, R] = assembleJacobian (src/Assembler.m:182-231)."""
+        plan = self.plan(space)
+        dev = _dev()
+        if not torch.is_tensor(solutionA):
+            solutionA = torch.from_numpy(np.ascontiguousarray(solutionA, dtype=np.float64))
+        sol = solutionA.to(dev, dtype=torch.float64).contiguous()
+        if sol.numel() < plan.n:
+            sol = torch.cat([sol, torch.zeros(plan.n - sol.numel(), dtype=torch.float64, device=dev)])
+        mat, keep = tables if tables is not None else self.material_tables(matLibData)
+        tags = self.Mesh.Tags_dev
+        ntag = int(mat.num_tags)
+        tmin, tmax = int(self.Mesh.RegionTags.min()), int(self.Mesh.RegionTags.max())
+        if tmin < 1 or tmax > ntag:
+            raise L.MpkError(L.MPK_ERR_INDEX, f"region tags [{tmin}, {tmax}] outside material table")
+        jv = torch.empty(max(plan.nnz, 1), dtype=torch.float64, device=dev) if calcJ else None
+        res = torch.empty(max(plan.n, 1), dtype=torch.float64, device=dev)
+        q = self.stiffness_quadrature()
+        L.check(L.lib.mpk_csc_jacobian(plan.handle, L.ptr(self.Mesh.P_dev), L.ptr(tags),
+                                       L.ptr(sol), C.byref(mat), C.byref(q), int(bool(calcJ)),
+                                       L.PRECISIONS[precision], L.ptr(jv), L.ptr(res),
+                                       L.stream_ptr()))
+        del keep
+        J = self._finish(plan, jv[: plan.nnz], False, drop_zeros) if calcJ else None
+        return J, res[: plan.n]
+
+    # snake_case aliases
+    assemble_stiffness = assembleStiffness
+    assemble_mass = assembleMass
+    assemble_mass_weighted = assembleMassWeighted
+    assemble_jacobian = assembleJacobian
+    assemble_frequency_operator = assembleFrequencyOperator

@@ -1,0 +1,136 @@
+"""MEX-signature drop-ins: the reference's kernel entry points on the GPU.
+
+Same names, argument lists, argument meaning, output layout and error
+identifier as the MEX files that ``Assembler.m`` calls:
+
+- ``assemble_curl_curl_kernel_mex(P, T, CellDofs, Signs, Nu, Qw, CurlRef)``
+  src/mex/assemble_curl_curl_kernel_mex.cpp:18-121 (7 inputs)
+- ``assemble_mass_kernel_mex(P, T, CellDofs, Signs, Coeffs, Qw, ValRef)``
+  src/mex/assemble_mass_kernel_mex.cpp:16-127 (7 inputs, Coeffs may be [])
+- ``assemble_jacobian_kernel_mex(P, T, CellDofs, Signs, Tags, SolA, Qw,
+  CurlRef, linNu, isNon, maxBSq, breaks, coefs, starts, counts, coeStarts,
+  calcJ)`` src/mex/assemble_jacobian_kernel_mex.cpp:7-153 (17 inputs)
+- ``sparse_gpu(I, J, V, m, n)``: ``valid = I>0 & J>0; sparse(...)``
+  (src/Assembler.m:37-38) on the GPU.
+
+Inputs are host arrays in MATLAB shapes (``P`` [3 x Nn], ``T`` [4 x Ne],
+...), read in column-major order exactly like ``mxGetPr``; outputs are host
+column vectors.  Work runs on ``cuda:0`` through libmpk_assembly.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _colmajor(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel(order="F"))
+
+
+def _ncols(x) -> int:
+    a = np.asarray(x)
+    return int(a.shape[1]) if a.ndim >= 2 else (1 if a.size else 0)
+
+
+def _dev(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(a).to(torch.device("cuda", torch.cuda.current_device()))
+
+
+def _arity(args, n):
+    if len(args) != n:
+        raise L.MpkError(L.MPK_ERR_ARGS, f"Expected {n} inputs.")
+
+
+def _out(n):
+    return torch.empty(max(n, 1), dtype=torch.float64,
+                       device=torch.device("cuda", torch.cuda.current_device()))
+
+
+def _quad(Qw, table, curl: bool) -> L.Quadrature:
+    qw = _colmajor(Qw)
+    tab = _colmajor(table)
+    if tab.size < 18 * qw.size:
+        raise L.MpkError(L.MPK_ERR_ARGS, "reference table smaller than 6 x 3 x numel(Qw)")
+    return L.make_quadrature(qw, None if curl else tab[: 18 * qw.size],
+                             tab[: 18 * qw.size] if curl else None)
+
+
+def assemble_curl_curl_kernel_mex(*args):
+    _arity(args, 7)
+    P, T, Dofs, Signs, Nu, Qw, CurlRef = args
+    ne = _ncols(T)
+    nn = _colmajor(P).size // 3
+    q = _quad(Qw, CurlRef, True)
+    d = [_dev(_colmajor(x)) for x in (P, T, Dofs, Signs, Nu)]
+    Io, Jo, Vo = _out(36 * ne), _out(36 * ne), _out(36 * ne)
+    L.check(L.lib.mpk_curl_curl_coo(L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]),
+                                    L.ptr(d[4]), C.byref(q), L.ptr(Io), L.ptr(Jo), L.ptr(Vo),
+                                    L.stream_ptr()))
+    return tuple(t[: 36 * ne].cpu().numpy() for t in (Io, Jo, Vo))
+
+
+def assemble_mass_kernel_mex(*args):
+    _arity(args, 7)
+    P, T, Dofs, Signs, Coeffs, Qw, ValRef = args
+    ne = _ncols(T)
+    nn = _colmajor(P).size // 3
+    q = _quad(Qw, ValRef, False)
+    d = [_dev(_colmajor(x)) for x in (P, T, Dofs, Signs)]
+    c = _colmajor(Coeffs) if Coeffs is not None else np.zeros(0)
+    cd = _dev(c) if c.size else None
+    Io, Jo, Vo = _out(36 * ne), _out(36 * ne), _out(36 * ne)
+    L.check(L.lib.mpk_mass_coo(L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]),
+                               L.ptr(cd), int(c.size), C.byref(q), L.ptr(Io), L.ptr(Jo),
+                               L.ptr(Vo), L.stream_ptr()))
+    return tuple(t[: 36 * ne].cpu().numpy() for t in (Io, Jo, Vo))
+
+
+def assemble_jacobian_kernel_mex(*args):
+    _arity(args, 17)
+    (P, T, Dofs, Signs, Tags, SolA, Qw, CurlRef, lin, isn, maxb, brk, coe, starts, counts,
+     coe_starts, calcJ) = args
+    ne = _ncols(T)
+    nn = _colmajor(P).size // 3
+    calc = bool(np.asarray(calcJ).ravel()[0])
+    q = _quad(Qw, CurlRef, True)
+    d = [_dev(_colmajor(x)) for x in (P, T, Dofs, Signs, Tags)]
+    sol = _colmajor(SolA)
+    sd = _dev(sol) if sol.size else None
+    tabs = [_dev(_colmajor(x)) if np.asarray(x).size else None
+            for x in (lin, isn, maxb, brk, coe, starts, counts, coe_starts)]
+    m = L.Material()
+    (m.linear_nu, m.is_nonlinear, m.max_bsq, m.breaks, m.coefs, m.spline_start, m.spline_count,
+     m.coef_start) = [L.ptr(t) for t in tabs]
+    m.num_tags = int(np.asarray(lin).size)
+    m.num_breaks = int(np.asarray(brk).size)
+    m.num_coefs = int(np.asarray(coe).size)
+    n36 = 36 * ne if calc else 0
+    Io, Jo, Vo = _out(n36), _out(n36), _out(n36)
+    Ri, Rv = _out(6 * ne), _out(6 * ne)
+    L.check(L.lib.mpk_jacobian_coo(L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]),
+                                   L.ptr(d[4]), L.ptr(sd), int(sol.size), C.byref(q), C.byref(m),
+                                   int(calc), L.ptr(Io), L.ptr(Jo), L.ptr(Vo), L.ptr(Ri),
+                                   L.ptr(Rv), L.stream_ptr()))
+    return (Io[:n36].cpu().numpy(), Jo[:n36].cpu().numpy(), Vo[:n36].cpu().numpy(),
+            Ri[: 6 * ne].cpu().numpy(), Rv[: 6 * ne].cpu().numpy())
+
+
+def sparse_gpu(I, J, V, m, n, drop_zeros: bool = True):
+    """Returns (colptr int64 [n+1], rowind int32 0-based [nnz], vals [nnz]) on the host."""
+    I, J, V = _colmajor(I), _colmajor(J), _colmajor(V)
+    nt = I.size
+    dev = torch.device("cuda", torch.cuda.current_device())
+    Id, Jd, Vd = (_dev(x) if nt else None for x in (I, J, V))
+    colptr = torch.empty(int(n) + 1, dtype=torch.int64, device=dev)
+    rowind = torch.empty(max(nt, 1), dtype=torch.int32, device=dev)
+    vals = torch.empty(max(nt, 1), dtype=torch.float64, device=dev)
+    nnz = C.c_int64(0)
+    L.check(L.lib.mpk_sparse_coo_to_csc(L.ptr(Id), L.ptr(Jd), L.ptr(Vd), nt, int(m), int(n),
+                                        int(bool(drop_zeros)), L.ptr(colptr), L.ptr(rowind),
+                                        L.ptr(vals), nt, C.byref(nnz), L.stream_ptr()))
+    k = int(nnz.value)
+    return colptr.cpu().numpy(), rowind[:k].cpu().numpy(), vals[:k].cpu().numpy()

@@ -1,0 +1,177 @@
+"""ctypes wrapper of the CPU oracle (oracle/libmpk_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.  Functions take
+numpy arrays in the reference's memory layouts (MATLAB column-major, which
+is C-order ``(N, k)`` for a ``[k x N]`` MATLAB matrix) and return numpy.
+See mpk_oracle.c for the reference file:line each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libmpk_oracle.so"
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def _load():
+    if not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "mpk_oracle.c").stat().st_mtime:
+        build()
+    return C.CDLL(str(LIB_PATH))
+
+
+_lib = _load()
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_i64 = C.c_int64
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+_lib.orc_generate_edges.restype = _i64
+_lib.orc_distribute_dofs.restype = _i64
+_lib.orc_sparse.restype = _i64
+
+
+def set_threads(n: int) -> None:
+    os.environ["OMP_NUM_THREADS"] = str(n)
+
+
+def generate_edges(T):
+    """Mesh.generateEdges: T (Ne,4) 1-based -> (Edges (E,2), T2E (Ne,6), Sign (Ne,6))."""
+    T, Tp = _i(T)
+    ne = T.shape[0]
+    edges = np.zeros((6 * ne, 2), dtype=np.int32)
+    t2e = np.zeros((ne, 6), dtype=np.int32)
+    sign = np.zeros((ne, 6), dtype=np.int8)
+    n = _lib.orc_generate_edges(Tp, ne, edges.ctypes.data_as(_ip), t2e.ctypes.data_as(_ip),
+                                sign.ctypes.data_as(C.POINTER(C.c_int8)))
+    return edges[:n].copy(), t2e, sign
+
+
+def distribute_dofs(t2e, nedge, active=None, offset=0):
+    t2e, tp = _i(t2e)
+    ne = t2e.shape[0]
+    emap = np.zeros(nedge, dtype=np.int32)
+    cell = np.zeros((ne, 6), dtype=np.int32)
+    act = None
+    if active is not None:
+        act = np.ascontiguousarray(active, dtype=np.uint8)
+    n = _lib.orc_distribute_dofs(tp, ne, nedge,
+                                 act.ctypes.data_as(C.POINTER(C.c_uint8)) if act is not None else None,
+                                 offset, emap.ctypes.data_as(_ip), cell.ctypes.data_as(_ip))
+    return emap, cell, int(n)
+
+
+def curl_curl(P, T, Dofs, Signs, Nu, Qw, CurlRef):
+    """assemble_curl_curl_kernel_mex: returns (I, J, V) of length 36*Ne."""
+    P, Pp = _d(P); T, Tp = _d(T); Dofs, Dp = _d(Dofs); Signs, Sp = _d(Signs)
+    Nu, Np = _d(Nu); Qw, Qp = _d(Qw); CurlRef, Cp = _d(CurlRef)
+    ne = T.size // 4
+    I = np.zeros(36 * ne); J = np.zeros(36 * ne); V = np.zeros(36 * ne)
+    _lib.orc_curl_curl_kernel(Pp, Tp, Dp, Sp, Np, Qp, _i64(Qw.size), Cp, _i64(ne),
+                              I.ctypes.data_as(_dp), J.ctypes.data_as(_dp), V.ctypes.data_as(_dp))
+    return I, J, V
+
+
+def mass(P, T, Dofs, Signs, Coeffs, Qw, ValRef):
+    """assemble_mass_kernel_mex (Coeffs may be None / [] -> sigma = 1)."""
+    P, Pp = _d(P); T, Tp = _d(T); Dofs, Dp = _d(Dofs); Signs, Sp = _d(Signs)
+    Qw, Qp = _d(Qw); ValRef, Vp = _d(ValRef)
+    ne = T.size // 4
+    if Coeffs is None:
+        cp, nc = None, 0
+    else:
+        Coeffs, cp = _d(Coeffs)
+        nc = Coeffs.size
+    I = np.zeros(36 * ne); J = np.zeros(36 * ne); V = np.zeros(36 * ne)
+    _lib.orc_mass_kernel(Pp, Tp, Dp, Sp, cp, _i64(nc), Qp, _i64(Qw.size), Vp, _i64(ne),
+                         I.ctypes.data_as(_dp), J.ctypes.data_as(_dp), V.ctypes.data_as(_dp))
+    return I, J, V
+
+
+def jacobian(P, T, Dofs, Signs, Tags, SolA, Qw, CurlRef, packed, calcJ=True):
+    """assemble_jacobian_kernel_mex: returns (I, J, V, R_idx, R_val)."""
+    P, Pp = _d(P); T, Tp = _d(T); Dofs, Dp = _d(Dofs); Signs, Sp = _d(Signs)
+    Tags, Tgp = _d(Tags); SolA, Ap = _d(SolA); Qw, Qp = _d(Qw); CurlRef, Cp = _d(CurlRef)
+    arrs = [_d(packed[k]) for k in ("linNu", "isNon", "maxB", "breaks", "coefs", "starts",
+                                    "counts", "coe_starts")]
+    ne = T.size // 4
+    n36 = 36 * ne if calcJ else 0
+    I = np.zeros(max(n36, 1)); J = np.zeros(max(n36, 1)); V = np.zeros(max(n36, 1))
+    Ri = np.zeros(6 * ne); Rv = np.zeros(6 * ne)
+    _lib.orc_jacobian_kernel(Pp, Tp, Dp, Sp, Tgp, Ap, Qp, _i64(Qw.size), Cp,
+                             *[a[1] for a in arrs], C.c_int(1 if calcJ else 0), _i64(ne),
+                             I.ctypes.data_as(_dp), J.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
+                             Ri.ctypes.data_as(_dp), Rv.ctypes.data_as(_dp))
+    return I[:n36], J[:n36], V[:n36], Ri, Rv
+
+
+def eval_nu(s, tags, packed):
+    s, sp = _d(s)
+    tags, tp = _i(tags)
+    arrs = [_d(packed[k]) for k in ("linNu", "isNon", "maxB", "breaks", "coefs", "starts",
+                                    "counts", "coe_starts")]
+    nu = np.zeros(s.size); dnu = np.zeros(s.size)
+    _lib.orc_eval_nu_many(sp, tp, _i64(s.size), *[a[1] for a in arrs],
+                          nu.ctypes.data_as(_dp), dnu.ctypes.data_as(_dp))
+    return nu, dnu
+
+
+def sparse(I, J, V, m, n, drop_zeros=True):
+    """sparse(I(valid), J(valid), V(valid), m, n): returns (colptr, rowind0, vals)."""
+    I, Ip = _d(I); J, Jp = _d(J); V, Vp = _d(V)
+    nt = I.size
+    colptr = np.zeros(n + 1, dtype=np.int64)
+    rowind = np.zeros(max(nt, 1), dtype=np.int64)
+    vals = np.zeros(max(nt, 1))
+    nnz = _lib.orc_sparse(Ip, Jp, Vp, _i64(nt), _i64(m), _i64(n), C.c_int(1 if drop_zeros else 0),
+                          colptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                          rowind.ctypes.data_as(C.POINTER(C.c_int64)), vals.ctypes.data_as(_dp))
+    if nnz < 0:
+        raise IndexError("sparse(): index exceeds matrix dimensions")
+    return colptr, rowind[:nnz].copy(), vals[:nnz].copy()
+
+
+def sparse_scipy(I, J, V, m, n, drop_zeros=True):
+    import scipy.sparse as sp
+    c, r, v = sparse(I, J, V, m, n, drop_zeros)
+    return sp.csc_matrix((v, r, c), shape=(m, n))
+
+
+def boundary_edges(T, edges):
+    T, Tp = _i(T)
+    edges, Ep = _i(edges)
+    out = np.zeros(edges.shape[0], dtype=np.uint8)
+    _lib.orc_boundary_edges(Tp, _i64(T.shape[0]), Ep, _i64(edges.shape[0]),
+                            out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out.astype(bool)
+
+
+# ---- composite helpers mirroring Assembler.m on the oracle ----------------
+
+def mesh_inputs(mesh, offset=0, active=None):
+    """Reference packing of a mesh: (P, T(double), CellDofs(double), Signs(double), nedge, ndof)."""
+    edges, t2e, sign = generate_edges(mesh.T)
+    emap, cell, nd = distribute_dofs(t2e, edges.shape[0], active, offset)
+    return dict(P=mesh.P, T=mesh.T.astype(np.float64), Dofs=cell.astype(np.float64),
+                Signs=sign.astype(np.float64), edges=edges, t2e=t2e, ndof=nd + offset,
+                cell=cell, sign=sign)

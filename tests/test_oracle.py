@@ -1,0 +1,344 @@
+"""Pins the CPU oracle before it is trusted as the checker of the GPU engine.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign or index, a transposed operand) fails: the reference's
+own known-answer test, closed forms computed independently with numpy,
+mathematical invariants of the discrete operators, and independent library
+routines (numpy.unique, scipy.sparse, scipy.interpolate.PPoly).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.interpolate import PPoly
+
+from oracle import oracle as O
+from magpackage_amd.fem_tables import EDGE_NODES, barycentric_moments
+from magpackage_amd.material import (MU0, create_linear, create_nonlinear, marrocco,
+                                     pack_material_data)
+from magpackage_amd.meshgen import kuhn_box, kuhn_counts
+from tests.helpers import (config1_mesh, config1_nu, gradient_matrix, grads, mass_tables,
+                           random_tets, rel_err, stiffness_tables, whitney_curls)
+
+REF_CURLS = np.array([[0, -2, 2], [2, 0, -2], [-2, 2, 0], [0, 0, 2], [0, -2, 0], [2, 0, 0]],
+                     dtype=np.float64)
+
+
+def _block(V, e):
+    """6x6 block K[r][c] of element e from the COO V (slot 36e + 6c + r)."""
+    return V[36 * e: 36 * e + 36].reshape(6, 6).T
+
+
+# ---------------------------------------------------------------- KAT ------
+def test_kat_reference_single_tet():
+    """tests/test_mex_assembly.m:31-73: unit tet, Dofs 1..6, Signs [1,-1,1,-1,1,-1],
+    Nu = 500.  Known answer K_ij = (500/6) (c_i . c_j) s_i s_j, max |dK| < 1e-13."""
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
+    T = np.array([[1, 2, 3, 4]], dtype=np.float64)
+    D = np.arange(1, 7, dtype=np.float64)[None]
+    S = np.array([[1, -1, 1, -1, 1, -1]], dtype=np.float64)
+    qw, curl, _ = stiffness_tables()
+    I, J, V = O.curl_curl(P, T, D, S, np.array([500.0]), qw, curl)
+    s = S[0]
+    K_exp = (500.0 / 6.0) * (REF_CURLS @ REF_CURLS.T) * np.outer(s, s)
+    assert np.max(np.abs(_block(V, 0) - K_exp)) < 1e-13
+    # triplet layout: slot 6c + r -> (dof r, dof c)
+    assert np.array_equal(I.reshape(6, 6), np.tile(np.arange(1, 7), (6, 1)))
+    assert np.array_equal(J.reshape(6, 6), np.repeat(np.arange(1, 7), 6).reshape(6, 6))
+    assert _block(V, 0)[0, 0] == pytest.approx(4000.0 / 6.0, rel=1e-15)
+
+
+# ---------------------------------------------------------- closed forms ---
+def test_curl_curl_closed_form_random_tets():
+    """K_e = nu |T| C_r . C_c s_r s_c with C = 2 grad l_a x grad l_b (numpy inverse)."""
+    P, T = random_tets(200, seed=3)
+    ne = T.shape[0]
+    rng = np.random.default_rng(4)
+    S = rng.choice([-1.0, 1.0], size=(ne, 6))
+    D = np.arange(1, 6 * ne + 1, dtype=np.float64).reshape(ne, 6)
+    nu = rng.uniform(0.5, 2.0, ne) * 1e3
+    qw, curl, _ = stiffness_tables()
+    _, _, V = O.curl_curl(P, T.astype(np.float64), D, S, nu, qw, curl)
+    for e in range(ne):
+        p = P[T[e] - 1]
+        C, det = whitney_curls(p)
+        K = nu[e] * abs(det) / 6.0 * (C @ C.T) * np.outer(S[e], S[e])
+        assert rel_err(_block(V, e), K) < 1e-12
+
+
+def _mass_closed(p, moments):
+    """sum_q w_q W_r . W_c |det| with W_ab = l_a grad l_b - l_b grad l_a and the
+    barycentric moments Q_xy = int l_x l_y / |T_ref-volume| of a rule."""
+    g, det = grads(p)
+    G = g @ g.T
+    M = np.zeros((6, 6))
+    for r, (a, b) in enumerate(EDGE_NODES):
+        for c, (k, l) in enumerate(EDGE_NODES):
+            M[r, c] = (moments[a, k] * G[b, l] - moments[a, l] * G[b, k]
+                       - moments[b, k] * G[a, l] + moments[b, l] * G[a, k])
+    return M * abs(det)
+
+
+def test_mass_closed_form_exact_and_rule():
+    P, T = random_tets(100, seed=5)
+    ne = T.shape[0]
+    rng = np.random.default_rng(6)
+    S = rng.choice([-1.0, 1.0], size=(ne, 6))
+    D = np.arange(1, 6 * ne + 1, dtype=np.float64).reshape(ne, 6)
+    sig = rng.uniform(1.0, 3.0, ne)
+    qw, val, q_pts = mass_tables(2)
+    _, _, V = O.mass(P, T.astype(np.float64), D, S, sig, qw, val)
+    # exact integral: int l_a l_b = |T| (1 + delta_ab) / 20, reference volume 1/6
+    exact = (np.ones((4, 4)) + np.eye(4)) / 120.0
+    # moments of the truncated 4-point rule (get_quadrature_data.m:20-27), numpy
+    lam = np.vstack([1.0 - q_pts.sum(axis=0), q_pts])
+    rule = (lam * qw) @ lam.T
+    worst_exact = worst_rule = 0.0
+    for e in range(ne):
+        p = P[T[e] - 1]
+        ss = np.outer(S[e], S[e]) * sig[e]
+        got = _block(V, e)
+        worst_exact = max(worst_exact, rel_err(got, _mass_closed(p, exact) * ss))
+        worst_rule = max(worst_rule, rel_err(got, _mass_closed(p, rule) * ss))
+    assert worst_rule < 1e-12
+    # the truncated constants move the reference off the exact integral by ~5e-9
+    assert 1e-10 < worst_exact < 1e-8
+
+
+def test_mass_sigma_rules():
+    """sigma = 1 when Coeffs is empty; |sigma| <= 1e-12 gives a zero block; linear in sigma."""
+    P, T = random_tets(20, seed=7)
+    ne = T.shape[0]
+    D = np.arange(1, 6 * ne + 1, dtype=np.float64).reshape(ne, 6)
+    S = np.ones((ne, 6))
+    qw, val, _ = mass_tables(2)
+    Tf = T.astype(np.float64)
+    _, _, V1 = O.mass(P, Tf, D, S, None, qw, val)
+    _, _, Vo = O.mass(P, Tf, D, S, np.ones(ne), qw, val)
+    assert np.array_equal(V1, Vo)
+    _, _, Vw = O.mass(P, Tf, D, S, np.ones(ne + 1), qw, val)  # wrong length -> sigma = 1
+    assert np.array_equal(V1, Vw)
+    sig = np.full(ne, 2.5)
+    sig[3] = 1e-12
+    sig[4] = -5e-13
+    _, _, V2 = O.mass(P, Tf, D, S, sig, qw, val)
+    assert np.all(V2[36 * 3: 36 * 5] == 0.0)
+    keep = np.ones(36 * ne, dtype=bool)
+    keep[36 * 3: 36 * 5] = False
+    assert rel_err(V2[keep], 2.5 * V1[keep]) < 1e-15
+
+
+# --------------------------------------------------- global invariants -----
+@pytest.fixture(scope="module")
+def small_system():
+    m = config1_mesh(6, seed=11)
+    mi = O.mesh_inputs(m)
+    nu = config1_nu(m)
+    qw, curl, _ = stiffness_tables()
+    I, J, V = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl)
+    n = mi["ndof"]
+    K = O.sparse_scipy(I, J, V, n, n)
+    return m, mi, nu, K
+
+
+def test_stiffness_invariants(small_system):
+    m, mi, nu, K = small_system
+    n = K.shape[0]
+    assert n == kuhn_counts(6, 6, 6)[2]
+    # symmetric
+    assert abs(K - K.T).max() <= 1e-12 * abs(K).max()
+    # K G = 0 for the discrete gradient (curl grad = 0)
+    G = gradient_matrix(mi["edges"], m.num_nodes)
+    assert abs(K @ G).max() <= 1e-9 * abs(K).max()
+    # positive semi-definite
+    w = np.linalg.eigvalsh(K.toarray())
+    assert w.min() > -1e-9 * w.max()
+    # energy identity 0.5 a'Ka = sum_e 0.5 nu_e |B_e|^2 |T_e| (tests/test_energy.m:35-47)
+    rng = np.random.default_rng(12)
+    a = rng.normal(size=n)
+    W = 0.0
+    for e in range(m.num_elems):
+        p = m.P[m.T[e] - 1]
+        C, det = whitney_curls(p)
+        B = (a[mi["cell"][e] - 1] * mi["sign"][e]) @ C
+        W += 0.5 * nu[e] * (B @ B) * abs(det) / 6.0
+    assert abs(0.5 * a @ (K @ a) - W) <= 1e-12 * abs(W)
+
+
+def test_mass_spd():
+    m = config1_mesh(5, seed=13)
+    mi = O.mesh_inputs(m)
+    qw, val, _ = mass_tables(2)
+    I, J, V = O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], None, qw, val)
+    n = mi["ndof"]
+    M = O.sparse_scipy(I, J, V, n, n).toarray()
+    assert np.allclose(M, M.T, rtol=0, atol=1e-15 * abs(M).max())
+    assert np.linalg.eigvalsh(M).min() > 0
+
+
+# ---------------------------------------------------------- Jacobian --------
+def _mat_lib_config1():
+    return {1: create_linear(1.0), 2: create_linear(1000.0)}
+
+
+def test_jacobian_linear_equals_stiffness(small_system):
+    """Linear materials: J == K bitwise and R == K a (MexMaterialUtils.hpp:30-34)."""
+    m, mi, nu, K = small_system
+    packed = pack_material_data(_mat_lib_config1())
+    qw, curl, _ = stiffness_tables()
+    n = mi["ndof"]
+    a = np.random.default_rng(14).normal(size=n)
+    I, J, V, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"],
+                                 m.RegionTags.astype(np.float64), a, qw, curl, packed)
+    Ik, Jk, Vk = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl)
+    assert np.array_equal(V, Vk) and np.array_equal(I, Ik) and np.array_equal(J, Jk)
+    c, r, v = O.sparse(Ri, np.ones_like(Ri), Rv, n, 1, drop_zeros=True)
+    R = np.zeros(n)
+    R[r] = v
+    assert rel_err(R, K @ a) < 1e-12
+    # residual-only mode leaves J empty and R unchanged
+    I0, J0, V0, Ri0, Rv0 = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"],
+                                      m.RegionTags.astype(np.float64), a, qw, curl, packed,
+                                      calcJ=False)
+    assert I0.size == 0 and np.array_equal(Rv0, Rv)
+
+
+def test_jacobian_is_residual_derivative():
+    """Nonlinear Marrocco iron: J = dR/da by central differences of the residual mode."""
+    m = config1_mesh(4, seed=15)
+    mi = O.mesh_inputs(m)
+    lib = {1: create_linear(1.0), 2: marrocco()}
+    packed = pack_material_data(lib)
+    qw, curl, _ = stiffness_tables()
+    n = mi["ndof"]
+    rng = np.random.default_rng(16)
+    a = rng.normal(size=n) * 0.02  # |B| ~ O(1) T on this mesh
+    tags = m.RegionTags.astype(np.float64)
+
+    def resid(x):
+        _, _, _, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, x, qw,
+                                     curl, packed, calcJ=False)
+        R = np.zeros(n)
+        np.add.at(R, (Ri - 1).astype(np.int64), Rv)
+        return R
+
+    I, J, V, _, _ = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, a, qw, curl,
+                               packed)
+    Jm = O.sparse_scipy(I, J, V, n, n)
+    for k in range(3):
+        d = rng.normal(size=n)
+        h = 1e-7 * np.linalg.norm(a) / np.linalg.norm(d)
+        fd = (resid(a + h * d) - resid(a - h * d)) / (2 * h)
+        assert rel_err(Jm @ d, fd) < 2e-5
+
+
+# ------------------------------------------------- B-H material eval --------
+def test_eval_nu_against_ppoly_and_clamps():
+    B = np.linspace(0, 2.0, 30)
+    H = B / (1000 * MU0) * (1 + B ** 2)  # saturating-ish curve
+    nl = create_nonlinear(B, H)
+    mar = marrocco()
+    lib = {1: create_linear(2.0), 2: nl, 3: mar}
+    pk = pack_material_data(lib)
+    rng = np.random.default_rng(17)
+    # tag 2: interior points vs scipy PPoly on the same pp tables
+    s = rng.uniform(0, nl.MaxBSq, 2000)
+    nu, dnu = O.eval_nu(s, np.full(s.size, 2), pk)
+    pp = PPoly(nl.SplineNu.coefs.T, nl.SplineNu.breaks)
+    ref_nu, ref_d = pp(s), pp.derivative()(s)
+    clamp = (ref_nu > nl.nu0) | (ref_nu < 1e-7)
+    assert rel_err(nu[~clamp], ref_nu[~clamp]) < 1e-13
+    assert rel_err(dnu[~clamp], ref_d[~clamp]) < 1e-12
+    # breaks themselves (binary-search boundaries)
+    sb = nl.SplineNu.breaks.copy()
+    nub, _ = O.eval_nu(sb, np.full(sb.size, 2), pk)
+    assert rel_err(nub, np.minimum(pp(sb), nl.nu0)) < 1e-13
+    # extrapolation beyond MaxBSq: linear with the end slope (MexMaterialUtils.hpp:70-76)
+    se = nl.MaxBSq + np.array([0.01, 0.5])
+    nue, due = O.eval_nu(se, np.full(2, 2), pk)
+    end_v, end_d = pp(nl.MaxBSq), pp.derivative()(nl.MaxBSq)
+    exp_v = end_v + end_d * (se - nl.MaxBSq)
+    exp_v_cl = np.where((exp_v > nl.nu0) | (exp_v < 1e-7), nl.nu0, exp_v)
+    assert rel_err(nue, exp_v_cl) < 1e-13
+    # linear material: (linNu, 0)
+    nul, dnul = O.eval_nu(np.array([0.0, 5.0]), np.array([1, 1]), pk)
+    assert np.all(nul == 1.0 / (2.0 * MU0)) and np.all(dnul == 0)
+    # Marrocco above the vacuum value clamps to (nu0, 0): nu(2.0) > nu0
+    nuc, duc = O.eval_nu(np.array([2.0, 3.0]), np.array([3, 3]), pk)
+    assert np.all(nuc == 1.0 / MU0) and np.all(duc == 0)
+    # a table that goes below 1e-7 clamps to (nu0, 0) too
+    neg = create_nonlinear(np.array([0.0, 1.0, 2.0, 3.0]), np.array([0.0, 1e-9, 2e-9, 3e-9]))
+    pk2 = pack_material_data({1: neg})
+    nun, dun = O.eval_nu(np.array([1.0]), np.array([1]), pk2)
+    assert nun[0] == neg.nu0 and dun[0] == 0
+
+
+# ---------------------------------------------------------- sparse() --------
+def test_sparse_against_scipy():
+    rng = np.random.default_rng(18)
+    m, n, nt = 40, 30, 3000
+    I = rng.integers(0, m + 1, nt).astype(np.float64)  # 0 = filtered out
+    J = rng.integers(0, n + 1, nt).astype(np.float64)
+    V = rng.integers(-3, 4, nt).astype(np.float64)     # integer values: order-free sums
+    c, r, v = O.sparse(I, J, V, m, n, drop_zeros=True)
+    keep = (I > 0) & (J > 0)
+    ref = sp.coo_matrix((V[keep], (I[keep] - 1, J[keep] - 1)), shape=(m, n)).tocsc()
+    ref.sum_duplicates()
+    ref.eliminate_zeros()
+    ref.sort_indices()
+    assert np.array_equal(c, ref.indptr) and np.array_equal(r, ref.indices)
+    assert np.array_equal(v, ref.data)
+    # without dropping: the structural pattern keeps the zero sums
+    c2, r2, v2 = O.sparse(I, J, V, m, n, drop_zeros=False)
+    assert r2.size >= r.size and np.count_nonzero(v2 == 0) == r2.size - r.size
+    # duplicates are summed in triplet order (floating point)
+    Vf = rng.normal(size=nt)
+    _, _, vf = O.sparse(I, J, Vf, m, n, drop_zeros=True)
+    reff = sp.coo_matrix((Vf[keep], (I[keep] - 1, J[keep] - 1)), shape=(m, n)).tocsc()
+    reff.sum_duplicates()
+    reff.sort_indices()
+    assert rel_err(vf, reff.data) < 1e-14
+    with pytest.raises(IndexError):
+        O.sparse(np.array([m + 1.0]), np.array([1.0]), np.array([1.0]), m, n)
+
+
+# ------------------------------------------------------ topology / DOFs -----
+def test_edges_against_numpy_unique():
+    mesh = kuhn_box(5, 4, 3, seed=19)
+    edges, t2e, sign = O.generate_edges(mesh.T)
+    raw = np.stack([mesh.T[:, [a for a, _ in EDGE_NODES]].ravel(),
+                    mesh.T[:, [b for _, b in EDGE_NODES]].ravel()], axis=1)
+    srt = np.sort(raw, axis=1)
+    u, inv = np.unique(srt, axis=0, return_inverse=True)
+    assert np.array_equal(edges, u)
+    assert np.array_equal(t2e.ravel(), inv.ravel() + 1)
+    assert np.array_equal(sign.ravel(), np.where(raw[:, 0] == srt[:, 0], 1, -1))
+    assert edges.shape[0] == kuhn_counts(5, 4, 3)[2]
+
+
+def test_boundary_edges_brute_force():
+    mesh = kuhn_box(3, 3, 2, seed=20)
+    edges, _, _ = O.generate_edges(mesh.T)
+    got = O.boundary_edges(mesh.T, edges)
+    # brute force: an edge is on the boundary iff all its nodes lie on the box surface
+    # and it belongs to a face seen once; for a Kuhn box the surface test suffices
+    P = mesh.P
+    onb = lambda x: np.isclose(x[:, 0], 0) | np.isclose(x[:, 0], 1) | np.isclose(x[:, 1], 0) \
+        | np.isclose(x[:, 1], 1) | np.isclose(x[:, 2], 0) | np.isclose(x[:, 2], 1)
+    a, b = P[edges[:, 0] - 1], P[edges[:, 1] - 1]
+    same_face = np.zeros(edges.shape[0], dtype=bool)
+    for ax in range(3):
+        for v in (0.0, 1.0):
+            same_face |= np.isclose(a[:, ax], v) & np.isclose(b[:, ax], v)
+    assert np.array_equal(got, same_face & onb(a) & onb(b))
+
+
+def test_distribute_dofs_restricted():
+    mesh = config1_mesh(4, seed=21)
+    edges, t2e, _ = O.generate_edges(mesh.T)
+    act = mesh.RegionTags == 2
+    emap, cell, nd = O.distribute_dofs(t2e, edges.shape[0], act, offset=7)
+    used = np.unique(t2e[act])
+    assert nd == used.size
+    assert np.array_equal(emap[used - 1], 7 + np.arange(1, nd + 1))
+    assert np.count_nonzero(emap) == nd
+    assert np.array_equal(cell, emap[t2e - 1])
