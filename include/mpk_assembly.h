@@ -200,6 +200,36 @@ int mpk_csc_drop_zeros(const mpk_plan* plan, const double* vals, int complex_val
                        int64_t* colptr_out, int32_t* rowind_out, double* vals_out,
                        int64_t* nnz_out, void* stream);
 
+/* ========================================================================== */
+/* 5. Linear algebra used by the transient Newton driver                      */
+/* ========================================================================== */
+/* No reference counterpart in the assembly path: the reference solves with
+ * MUMPS / backslash (src/solver/LinearSolver.m:22-46), whose binaries are
+ * absent.  These operate on a structural CSC pattern of a symmetric matrix
+ * (device arrays). */
+
+/* y = A x for symmetric A stored as CSC (== CSR of A). */
+int mpk_spmv_sym(const int32_t* colptr, const int32_t* rowind, const double* vals, int64_t n,
+                 const double* x, double* y, void* stream);
+
+/* out = c0*v0 + c1*v1 + c2*v2 + c3*v3 over nnz values of one pattern; v1..v3 may be NULL. */
+int mpk_csc_combine(int64_t nnz, double c0, const double* v0, double c1, const double* v1,
+                    double c2, const double* v2, double c3, const double* v3, double* out,
+                    void* stream);
+
+/* Doubles of device workspace mpk_pcg_solve needs for n unknowns. */
+int mpk_pcg_work_size(int64_t n, int64_t* size_out);
+
+/* Jacobi-PCG on the Dirichlet-applied operator of BoundaryCondition.applyDirichlet
+ * (src/BoundaryCondition.m:81-112): fixed [n] (uint8, NULL = none) removes the
+ * rows and columns of fixed DOFs and puts a unit diagonal back; b's fixed
+ * entries are treated as 0.  Starts from x = 0; stops when |r|/|b| <= rel_tol
+ * (checked every check_every iterations) or after max_iter.  Synchronises. */
+int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* vals, int64_t n,
+                  const uint8_t* fixed, const double* b, double* x, double* work,
+                  double rel_tol, int max_iter, int check_every, int* iters_out,
+                  double* rel_res_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,11 @@ _SIGS = {
                                        _i32),
     "mpk_csc_jacobian": ([_plan_p, _vp, _vp, _vp, _pm, _pq, _i32, _i32, _vp, _vp, _vp], _i32),
     "mpk_csc_drop_zeros": ([_plan_p, _vp, _i32, _vp, _vp, _vp, _pi64, _vp], _i32),
+    "mpk_spmv_sym": ([_vp, _vp, _vp, _i64, _vp, _vp, _vp], _i32),
+    "mpk_csc_combine": ([_i64, _d, _vp, _d, _vp, _d, _vp, _d, _vp, _vp, _vp], _i32),
+    "mpk_pcg_work_size": ([_i64, _pi64], _i32),
+    "mpk_pcg_solve": ([_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _d, _i32, _i32,
+                       C.POINTER(C.c_int), C.POINTER(C.c_double), _vp], _i32),
 }
 for _name, (_args, _res) in _SIGS.items():
     _f = getattr(lib, _name)

@@ -205,7 +205,8 @@ class Plan:
                                          L.ptr(rowind), L.ptr(vout), C.byref(nnz),
                                          L.stream_ptr()))
         k = int(nnz.value)
-        return CscMatrix((self.n, self.n), colptr, rowind[:k], vout[:k])
+        return CscMatrix((self.n, self.n), colptr, rowind[:k],
+                         vout[: 2 * k] if complex_vals else vout[:k])
 
     def __del__(self):
         p = getattr(self, "_p", None)

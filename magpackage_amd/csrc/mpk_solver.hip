@@ -1,0 +1,349 @@
+// Sparse linear algebra for the transient Newton driver on gfx950.
+//
+// The reference solves each Newton step with MUMPS (src/solver/LinearSolver.m:22-46),
+// whose binaries are absent; both arms of the learned/plain comparison use this
+// solver instead.  The system is the Jacobian after BoundaryCondition.applyDirichlet
+// (src/BoundaryCondition.m:81-112): rows and columns of fixed DOFs removed and a unit
+// diagonal put back.  That operator is applied matrix-free from the assembled
+// (symmetric, structural-pattern) CSC values and a fixed-DOF mask, so the pattern is
+// never rebuilt.
+//
+// Jacobi-preconditioned conjugate gradients with every scalar kept on the device:
+// one iteration is 4 kernels (SpMV+dot, alpha, update+dot, beta+direction) and
+// needs no host round trip; the host checks convergence every `check_every`
+// iterations.  Dot products reduce per block into a fixed slot array and the
+// final sum runs in slot order in one block, so results are deterministic.
+#include <cmath>
+
+#include "mpk_common.hpp"
+
+namespace mpk {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kRowLanes = 8;             // lanes cooperating on one CSR row
+constexpr int kMaxPartials = 4096;       // reduction slots (grid-stride kernels)
+
+// y = A_bc x where A is symmetric and stored CSC (== CSR of A).  Fixed rows and
+// columns are removed and replaced by the identity.
+__global__ void __launch_bounds__(kB)
+k_spmv_dot(const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+           const double* __restrict__ vals, const uint8_t* __restrict__ fixed, int64_t n,
+           const double* __restrict__ x, double* __restrict__ y, double* __restrict__ partial) {
+    __shared__ double red[kB / 64];
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / kRowLanes);
+    double acc_dot = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * (kB / kRowLanes) + threadIdx.x / kRowLanes; i < n;
+         i += rows_per_grid) {
+        double s = 0.0;
+        const bool fi = fixed && fixed[i];
+        if (!fi) {
+            for (int k = colptr[i] + lane; k < colptr[i + 1]; k += kRowLanes) {
+                const int j = rowind[k];
+                if (!(fixed && fixed[j])) s += vals[k] * x[j];
+            }
+        }
+        // reduce the row over its kRowLanes lanes (fixed butterfly order)
+#pragma unroll
+        for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kRowLanes);
+        if (fi) s = x[i];
+        if (lane == 0) {
+            y[i] = s;
+            acc_dot += s * x[i];
+        }
+    }
+    // block reduction of p.q
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc_dot += __shfl_xor(acc_dot, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc_dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kB / 64; ++w) t += red[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// Sum of nparts partials in slot order (one block of 256 lanes, fixed tree).
+__device__ double block_sum_fixed(const double* __restrict__ partial, int nparts) {
+    __shared__ double sh[kB];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kB) t += partial[i];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// scal[0] = rz, scal[1] = alpha, scal[2] = beta, scal[3] = |r|^2
+__global__ void __launch_bounds__(kB) k_alpha(const double* __restrict__ partial, int nparts,
+                                              double* __restrict__ scal) {
+    const double pq = block_sum_fixed(partial, nparts);
+    if (threadIdx.x == 0) scal[1] = (pq != 0.0) ? scal[0] / pq : 0.0;
+}
+
+__global__ void __launch_bounds__(kB)
+k_update(int64_t n, const double* __restrict__ scal, const double* __restrict__ p,
+         const double* __restrict__ q, const double* __restrict__ dinv, double* __restrict__ x,
+         double* __restrict__ r, double* __restrict__ z, double* __restrict__ partial_rz,
+         double* __restrict__ partial_rr) {
+    __shared__ double red[2][kB / 64];
+    const double alpha = scal[1];
+    double rz = 0.0, rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const double zi = ri * dinv[i];
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        rz += __shfl_xor(rz, off, 64);
+        rr += __shfl_xor(rr, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = rz;
+        red[1][threadIdx.x >> 6] = rr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kB / 64; ++w) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        partial_rz[blockIdx.x] = a;
+        partial_rr[blockIdx.x] = b;
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_beta(const double* __restrict__ prz,
+                                             const double* __restrict__ prr, int nparts,
+                                             double* __restrict__ scal) {
+    const double rz_new = block_sum_fixed(prz, nparts);
+    const double rr = block_sum_fixed(prr, nparts);
+    if (threadIdx.x == 0) {
+        scal[2] = (scal[0] != 0.0) ? rz_new / scal[0] : 0.0;
+        scal[0] = rz_new;
+        scal[3] = rr;
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_direction(int64_t n, const double* __restrict__ scal,
+                                                  const double* __restrict__ z,
+                                                  double* __restrict__ p) {
+    const double beta = scal[2];
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB)
+        p[i] = z[i] + beta * p[i];
+}
+
+// Setup: dinv = 1/diag (1 on fixed rows), r = b (0 on fixed), z = dinv r, p = z,
+// partials of r.z and r.r.
+__global__ void __launch_bounds__(kB)
+k_setup(const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+        const double* __restrict__ vals, const uint8_t* __restrict__ fixed, int64_t n,
+        const double* __restrict__ b, double* __restrict__ dinv, double* __restrict__ r,
+        double* __restrict__ z, double* __restrict__ p, double* __restrict__ x,
+        double* __restrict__ prz, double* __restrict__ prr) {
+    __shared__ double red[2][kB / 64];
+    double rz = 0.0, rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+        const bool fi = fixed && fixed[i];
+        double d = 1.0;
+        if (!fi) {
+            d = 0.0;
+            for (int k = colptr[i]; k < colptr[i + 1]; ++k)
+                if (rowind[k] == i) d += vals[k];
+        }
+        const double di = (d != 0.0) ? 1.0 / d : 1.0;
+        dinv[i] = di;
+        const double ri = fi ? 0.0 : b[i];
+        x[i] = 0.0;
+        r[i] = ri;
+        z[i] = ri * di;
+        p[i] = ri * di;
+        rz += ri * ri * di;
+        rr += ri * ri;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        rz += __shfl_xor(rz, off, 64);
+        rr += __shfl_xor(rr, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = rz;
+        red[1][threadIdx.x >> 6] = rr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int w = 0; w < kB / 64; ++w) {
+            a += red[0][w];
+            c += red[1][w];
+        }
+        prz[blockIdx.x] = a;
+        prr[blockIdx.x] = c;
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_setup_scal(const double* __restrict__ prz,
+                                                   const double* __restrict__ prr, int nparts,
+                                                   double* __restrict__ scal) {
+    const double rz = block_sum_fixed(prz, nparts);
+    const double rr = block_sum_fixed(prr, nparts);
+    if (threadIdx.x == 0) {
+        scal[0] = rz;
+        scal[1] = 0.0;
+        scal[2] = 0.0;
+        scal[3] = rr;
+    }
+}
+
+// y = A_bc x (the Dirichlet-applied operator), no dot product.
+__global__ void __launch_bounds__(kB)
+k_spmv(const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+       const double* __restrict__ vals, const uint8_t* __restrict__ fixed, int mask_mode,
+       int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / kRowLanes);
+    for (int64_t i = (int64_t)blockIdx.x * (kB / kRowLanes) + threadIdx.x / kRowLanes; i < n;
+         i += rows_per_grid) {
+        double s = 0.0;
+        for (int k = colptr[i] + lane; k < colptr[i + 1]; k += kRowLanes) s += vals[k] * x[rowind[k]];
+#pragma unroll
+        for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kRowLanes);
+        if (lane == 0) y[i] = s;
+    }
+}
+
+// out = sum_k coef[k] * vals_k (same pattern), up to 4 inputs.
+__global__ void __launch_bounds__(kB)
+k_axpby4(int64_t nnz, double c0, const double* __restrict__ v0, double c1,
+         const double* __restrict__ v1, double c2, const double* __restrict__ v2, double c3,
+         const double* __restrict__ v3, double* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * kB) {
+        double s = c0 * v0[i];
+        if (v1) s += c1 * v1[i];
+        if (v2) s += c2 * v2[i];
+        if (v3) s += c3 * v3[i];
+        out[i] = s;
+    }
+}
+
+int grid_parts(int64_t n, int per_block) {
+    int64_t g = (n + per_block - 1) / per_block;
+    if (g > kMaxPartials) g = kMaxPartials;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+}  // namespace
+}  // namespace mpk
+
+using namespace mpk;
+
+extern "C" {
+
+int mpk_spmv_sym(const int32_t* colptr, const int32_t* rowind, const double* vals, int64_t n,
+                 const double* x, double* y, void* stream) {
+    clear_error();
+    MPK_REQUIRE(n >= 0, MPK_ERR_ARGS, "negative size");
+    if (n == 0) return MPK_OK;
+    MPK_REQUIRE(colptr && rowind && vals && x && y, MPK_ERR_ARGS, "NULL argument");
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_spmv, dim3(grid_parts(n, kB / kRowLanes)), dim3(kB), 0, st, colptr,
+                       rowind, vals, nullptr, 0, n, x, y);
+    MPK_HIP_TRY(hipGetLastError());
+    return MPK_OK;
+}
+
+int mpk_csc_combine(int64_t nnz, double c0, const double* v0, double c1, const double* v1,
+                    double c2, const double* v2, double c3, const double* v3, double* out,
+                    void* stream) {
+    clear_error();
+    MPK_REQUIRE(nnz >= 0, MPK_ERR_ARGS, "negative size");
+    if (nnz == 0) return MPK_OK;
+    MPK_REQUIRE(v0 && out, MPK_ERR_ARGS, "NULL argument");
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_axpby4, dim3(grid_parts(nnz, kB)), dim3(kB), 0, st, nnz, c0, v0, c1, v1,
+                       c2, v2, c3, v3, out);
+    MPK_HIP_TRY(hipGetLastError());
+    return MPK_OK;
+}
+
+int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* vals, int64_t n,
+                  const uint8_t* fixed, const double* b, double* x, double* work,
+                  double rel_tol, int max_iter, int check_every, int* iters_out,
+                  double* rel_res_out, void* stream) {
+    clear_error();
+    MPK_REQUIRE(n >= 0 && max_iter >= 0 && iters_out && rel_res_out, MPK_ERR_ARGS,
+                "bad argument");
+    *iters_out = 0;
+    *rel_res_out = 0.0;
+    if (n == 0) return MPK_OK;
+    MPK_REQUIRE(colptr && rowind && vals && b && x && work, MPK_ERR_ARGS, "NULL argument");
+    if (check_every < 1) check_every = 1;
+    hipStream_t st = as_stream(stream);
+    // work layout: dinv, r, z, p, q (n each), partials (3 * kMaxPartials), scal (8)
+    double* dinv = work;
+    double* r = dinv + n;
+    double* z = r + n;
+    double* p = z + n;
+    double* q = p + n;
+    double* pa = q + n;
+    double* pb = pa + kMaxPartials;
+    double* pc = pb + kMaxPartials;
+    double* scal = pc + kMaxPartials;
+    const int g_vec = grid_parts(n, kB);
+    const int g_row = grid_parts(n, kB / kRowLanes);
+    hipLaunchKernelGGL(k_setup, dim3(g_vec), dim3(kB), 0, st, colptr, rowind, vals, fixed, n, b,
+                       dinv, r, z, p, x, pb, pc);
+    hipLaunchKernelGGL(k_setup_scal, dim3(1), dim3(kB), 0, st, pb, pc, g_vec, scal);
+    MPK_HIP_TRY(hipGetLastError());
+    double h[4];
+    MPK_HIP_TRY(hipMemcpyAsync(h, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MPK_HIP_TRY(hipStreamSynchronize(st));
+    const double r0 = std::sqrt(h[3]);
+    if (r0 == 0.0) return MPK_OK;
+    int it = 0;
+    double rel = 1.0;
+    while (it < max_iter) {
+        const int batch = (max_iter - it) < check_every ? (max_iter - it) : check_every;
+        for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL(k_spmv_dot, dim3(g_row), dim3(kB), 0, st, colptr, rowind, vals,
+                               fixed, n, p, q, pa);
+            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, pa, g_row, scal);
+            hipLaunchKernelGGL(k_update, dim3(g_vec), dim3(kB), 0, st, n, scal, p, q, dinv, x, r,
+                               z, pb, pc);
+            hipLaunchKernelGGL(k_beta, dim3(1), dim3(kB), 0, st, pb, pc, g_vec, scal);
+            hipLaunchKernelGGL(k_direction, dim3(g_vec), dim3(kB), 0, st, n, scal, z, p);
+        }
+        MPK_HIP_TRY(hipGetLastError());
+        it += batch;
+        MPK_HIP_TRY(hipMemcpyAsync(h, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        MPK_HIP_TRY(hipStreamSynchronize(st));
+        rel = std::sqrt(h[3]) / r0;
+        if (!(rel > rel_tol)) break;  // also stops on NaN
+    }
+    *iters_out = it;
+    *rel_res_out = rel;
+    return MPK_OK;
+}
+
+int mpk_pcg_work_size(int64_t n, int64_t* size_out) {
+    clear_error();
+    MPK_REQUIRE(n >= 0 && size_out, MPK_ERR_ARGS, "bad argument");
+    *size_out = 5 * n + 3 * (int64_t)kMaxPartials + 8;
+    return MPK_OK;
+}
+
+}  // extern "C"
