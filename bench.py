@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=40)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="after timing, run a plain 512 MiB elementwise copy (reads and writes "
+                         "512 MiB) so HBM counters can be scaled against a known size")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -178,6 +181,15 @@ def main():
     achieved = alg / (ms_per * 1e-3) / 1e9
     log(f"[bench] {args.steps} steps: {ms_per:.4f} ms/step (events), wall {1e3 * (tw1 - tw0):.1f} ms,"
         f" {value:.4g} tets/s, {achieved:.0f} GB/s algorithmic")
+
+    if args.calib:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 3 x 512 MiB read + 512 MiB write")
+        del a, b
 
     cpu = None
     if not args.no_cpu:

@@ -45,6 +45,26 @@ __host__ __device__ constexpr int edge_b(int k) {
     return k == 0 ? 1 : (k == 1 ? 2 : (k == 2 ? 3 : (k == 3 ? 2 : 3)));
 }
 
+// Index of local edge (a, b), a < b.
+__host__ __device__ constexpr int edge_index(int a, int b) {
+    return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5);
+}
+// Node permutation that makes local edge c the edge (0,1) of a new frame,
+// the other two nodes in ascending order:
+//   c=0 (0,1,2,3)  c=1 (0,2,1,3)  c=2 (0,3,1,2)  c=3 (1,2,0,3)  c=4 (1,3,0,2)  c=5 (2,3,0,1)
+__host__ __device__ constexpr int perm_node(int c, int i) {
+    return (int)(((c == 0 ? 0x3210u : c == 1 ? 0x3120u : c == 2 ? 0x2130u
+                 : c == 3 ? 0x3021u : c == 4 ? 0x2031u : 0x1032u) >> (4 * i)) & 0xfu);
+}
+// Original local edge of frame edge rp.
+__host__ __device__ constexpr int orig_edge(int c, int rp) {
+    const int a = perm_node(c, edge_a(rp)), b = perm_node(c, edge_b(rp));
+    return a < b ? edge_index(a, b) : edge_index(b, a);
+}
+static_assert(orig_edge(0, 0) == 0 && orig_edge(5, 0) == 5 && orig_edge(5, 5) == 0 &&
+                  orig_edge(2, 5) == 3 && orig_edge(4, 3) == 2,
+              "permutation tables");
+
 // RAII device buffer for library-internal scratch.
 struct DevBuf {
     void* p = nullptr;

@@ -23,4 +23,26 @@ struct mpk_plan {
     uint2* inc_pos = nullptr;     // [ninc], byte r = in-column position of row r (0xFF: none)
     int4* T = nullptr;            // [ne], 0-based node ids
     mpk::DevBuf buf_colptr, buf_rowind, buf_incptr, buf_incslot, buf_incpos, buf_T;
+
+    // Closed-form (frame) kernels: one 16-byte record per incidence.
+    //   x = element id
+    //   y = c | orientation bits f_r' (bits 8..13) | pos'_0 << 16 | pos'_1 << 24
+    //   z = pos'_2 | pos'_3 << 8 | pos'_4 << 16 | pos'_5 << 24
+    //   w = 0
+    // The frame orders the element's nodes (lo, hi, o1, o2): lo/hi are the
+    // column edge's endpoints by global id (the same two nodes for every
+    // incidence of the column), o1/o2 the other two in local order.  r' runs
+    // over the frame's local edges, pos'_r' is the in-column position of that
+    // edge's row (0xff: inactive) and f_r' = 1 when the frame edge runs from
+    // the lower to the higher global node id.
+    // Records are stored k-major per tile: the k-th incidence of the tile's
+    // column `lane` sits at frame_base[tile] + 64 k + lane, so a wave reads the
+    // k-th records of its 64 columns as one contiguous block.
+    uint4* inc_frame = nullptr;
+    int32_t* frame_base = nullptr;  // [ntiles+1]
+    mpk::DevBuf buf_incframe, buf_framebase;
+    // Node coordinates padded to 32 bytes (x, y, z, 0), refreshed by every call
+    // of a frame kernel from the caller's P.
+    double4* P4 = nullptr;
+    mpk::DevBuf buf_P4;
 };

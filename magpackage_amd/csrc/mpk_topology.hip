@@ -317,6 +317,54 @@ __global__ void k_plan_pos(const uint64_t* __restrict__ keys, const uint32_t* __
     posb[8 * (int64_t)ii + r] = (uint8_t)pos;
 }
 
+// Frame-record slots per tile: 64 x (largest incidence count of its columns).
+__global__ void k_frame_tile_size(const int32_t* __restrict__ inc_ptr, int64_t n,
+                                  int32_t* __restrict__ size) {
+    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t j0 = t * kTileCols;
+    if (j0 >= n) return;
+    const int64_t j1 = j0 + kTileCols < n ? j0 + kTileCols : n;
+    int m = 0;
+    for (int64_t j = j0; j < j1; ++j) m = max(m, inc_ptr[j + 1] - inc_ptr[j]);
+    size[t] = m * kTileCols;
+}
+
+// Frame records of the closed-form kernels (layout in mpk_plan.hpp), one
+// thread per column, written k-major into the column's tile.
+__global__ void k_plan_frame(const int32_t* __restrict__ inc_ptr,
+                             const uint32_t* __restrict__ slot, const uint2* __restrict__ pos,
+                             const int4* __restrict__ T, const int32_t* __restrict__ frame_base,
+                             int64_t n, uint4* __restrict__ rec) {
+    const int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (j >= n) return;
+    const int64_t base = frame_base[j / kTileCols] + (j % kTileCols);
+    for (int k = inc_ptr[j]; k < inc_ptr[j + 1]; ++k) {
+        const uint32_t s = slot[k];
+        const uint32_t e = s / 6u;
+        const int c = (int)(s - 6u * e);
+        const uint2 pw = pos[k];
+        const int4 t = T[e];
+        const int nd[4] = {t.x, t.y, t.z, t.w};
+        // frame nodes (lo, hi, o1, o2) as local indices
+        int fr[4];
+        const int a = edge_a(c), b = edge_b(c);
+        fr[0] = nd[a] <= nd[b] ? a : b;
+        fr[1] = nd[a] <= nd[b] ? b : a;
+        fr[2] = perm_node(c, 2);
+        fr[3] = perm_node(c, 3);
+        uint32_t fmask = 0, p[6];
+        for (int rp = 0; rp < 6; ++rp) {
+            const int u = fr[edge_a(rp)], v = fr[edge_b(rp)];
+            if (nd[u] < nd[v]) fmask |= 1u << rp;
+            const int r = u < v ? edge_index(u, v) : edge_index(v, u);
+            p[rp] = r < 4 ? ((pw.x >> (8 * r)) & 0xffu) : ((pw.y >> (8 * (r - 4))) & 0xffu);
+        }
+        rec[base + (int64_t)(k - inc_ptr[j]) * kTileCols] =
+            make_uint4(e, (uint32_t)c | (fmask << 8) | (p[0] << 16) | (p[1] << 24),
+                       p[2] | (p[3] << 8) | (p[4] << 16) | (p[5] << 24), 0u);
+    }
+}
+
 __global__ void k_tile_max(const int32_t* __restrict__ colptr, int64_t n,
                            int* __restrict__ maxv) {
     const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
@@ -667,6 +715,40 @@ int mpk_plan_create(const int32_t* T, const int32_t* cell_dofs, int64_t ne, int6
             set_error("a column holds more than 255 entries (engine limit)");
             return fail(MPK_ERR_LIMIT);
         }
+    }
+    if (plan->ninc > 0) {
+        const int64_t ntiles = (ndofs + kTileCols - 1) / kTileCols;
+        DevBuf tsize;
+        if (tsize.alloc((ntiles + 1) * 4) != hipSuccess ||
+            plan->buf_framebase.alloc((ntiles + 1) * 4) != hipSuccess) {
+            set_error("hipMalloc failed for frame tiles");
+            return fail(MPK_ERR_HIP);
+        }
+        (void)hipMemsetAsync(tsize.p, 0, (ntiles + 1) * 4, st);
+        hipLaunchKernelGGL(k_frame_tile_size, dim3(grid_for(ntiles, kB)), dim3(kB), 0, st,
+                           plan->inc_ptr, ndofs, tsize.as<int32_t>());
+        plan->frame_base = plan->buf_framebase.as<int32_t>();
+        if (int rc = exclusive_scan(tsize.as<int32_t>(), plan->frame_base, ntiles + 1, st))
+            return fail(rc);
+        int32_t nslots = 0;
+        if (int rc = to_host(&nslots, plan->frame_base + ntiles, 1, st)) return fail(rc);
+        if (plan->buf_incframe.alloc((size_t)nslots * 16) != hipSuccess) {
+            set_error("hipMalloc failed for frame records");
+            return fail(MPK_ERR_HIP);
+        }
+        plan->inc_frame = plan->buf_incframe.as<uint4>();
+        hipLaunchKernelGGL(k_plan_frame, dim3(grid_for(ndofs, kB)), dim3(kB), 0, st,
+                           plan->inc_ptr, plan->inc_slot, plan->inc_pos, plan->T,
+                           plan->frame_base, ndofs, plan->inc_frame);
+    }
+    if (plan->buf_P4.alloc((size_t)(nn > 0 ? nn : 1) * 32) != hipSuccess) {
+        set_error("hipMalloc failed for padded coordinates");
+        return fail(MPK_ERR_HIP);
+    }
+    plan->P4 = plan->buf_P4.as<double4>();
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+        set_error("plan frame kernel failed");
+        return fail(MPK_ERR_HIP);
     }
     *plan_out = plan;
     return MPK_OK;

@@ -161,8 +161,7 @@ def test_plan_pattern_matches_sparse(cfg1):
                           qw, curl)
     c, r, _ = O.sparse(I, J, V, mi["ndof"], mi["ndof"], drop_zeros=False)
     colptr, rowind = plan.pattern()
-    assert plan.nnz == r.size == 938060  # nnz(K) of the 20^3 Kuhn box
-    assert np.array_equal(colptr.cpu().numpy(), c)
+    assert plan.nnz == r.size == 938060  # nnz(K) of the 20^3 Kuhn box    assert np.array_equal(colptr.cpu().numpy(), c)
     assert np.array_equal(rowind.cpu().numpy(), r)
 
 
