@@ -160,6 +160,10 @@ int mpk_plan_destroy(mpk_plan* plan);
 /* nnz, number of columns, elements, incidences and the largest LDS tile. */
 int mpk_plan_info(const mpk_plan* plan, int64_t* nnz, int64_t* ncols, int64_t* ne,
                   int64_t* n_incidences, int64_t* max_tile_nnz);
+/* *ring != 0 when the closed-form (FP64, FP32_LOCAL) kernels walk each
+ * column's elements in rotation order around the edge (every column a simple
+ * ring or chain, < 2^24 elements and nodes); else element order. */
+int mpk_plan_ring_order(const mpk_plan* plan, int* ring);
 /* Device pointers owned by the plan: colptr [n+1], rowind [nnz] (0-based). */
 int mpk_plan_pattern(const mpk_plan* plan, const int32_t** colptr, const int32_t** rowind);
 

@@ -176,6 +176,9 @@ class Plan:
                                     C.byref(ninc), C.byref(mt)))
         self.nnz, self.n, self.ne = nnz.value, nc.value, ne.value
         self.n_incidences, self.max_tile_nnz = ninc.value, mt.value
+        ring = C.c_int()
+        L.check(L.lib.mpk_plan_ring_order(self._p, C.byref(ring)))
+        self.ring_order = bool(ring.value)
         cp, ri = C.c_void_p(), C.c_void_p()
         L.check(L.lib.mpk_plan_pattern(self._p, C.byref(cp), C.byref(ri)))
         self._colptr_addr, self._rowind_addr = cp.value, ri.value

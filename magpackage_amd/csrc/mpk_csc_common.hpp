@@ -1,0 +1,100 @@
+// Shared pieces of the column-owner CSC assembly kernels: kernel argument
+// structs, small vector math and the XCD-aware tile mapping.
+#pragma once
+
+#include <cmath>
+
+#include "mpk_plan.hpp"
+
+namespace mpk {
+namespace csc {
+
+constexpr int kW = kTileCols;  // lanes per block == columns per tile
+constexpr int kMaxLdsBytes = 64 * 1024;
+constexpr int kMatLds = 2048;  // B-H table doubles staged in LDS
+
+struct CscQuad {
+    int nq;
+    double qw[8];
+    double ref[144];
+    double wsum;    // sum of weights
+    double qd, qo;  // symmetric barycentric moments (diag / off-diag mean)
+};
+
+struct TileCtx {
+    const int32_t* __restrict__ colptr;
+    const int32_t* __restrict__ inc_ptr;
+    const uint32_t* __restrict__ inc_slot;
+    const uint2* __restrict__ inc_pos;
+    const uint4* __restrict__ inc_frame;
+    const uint4* __restrict__ inc_ring;
+    const int32_t* __restrict__ frame_base;
+    const int4* __restrict__ T;
+    const double4* __restrict__ P4;
+    const int32_t* __restrict__ rowind;
+    int64_t n;
+};
+
+__device__ __forceinline__ int pos_of(uint2 pw, int r) {
+    return r < 4 ? (int)((pw.x >> (8 * r)) & 0xffu) : (int)((pw.y >> (8 * (r - 4))) & 0xffu);
+}
+
+template <class T>
+__device__ __forceinline__ T sel6(int c, T v0, T v1, T v2, T v3, T v4, T v5) {
+    return c == 0 ? v0 : c == 1 ? v1 : c == 2 ? v2 : c == 3 ? v3 : c == 4 ? v4 : v5;
+}
+
+struct Vec3 {
+    double x, y, z;
+};
+__device__ __forceinline__ Vec3 sub(Vec3 a, Vec3 b) { return Vec3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ Vec3 cross(Vec3 a, Vec3 b) {
+    return Vec3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+struct Vec3f {
+    float x, y, z;
+};
+__device__ __forceinline__ Vec3f tof(Vec3 v) { return Vec3f{(float)v.x, (float)v.y, (float)v.z}; }
+__device__ __forceinline__ float dotf(Vec3f a, Vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+// Remap so that each XCD sweeps one contiguous range of column tiles: the
+// elements and nodes shared by neighbouring tiles then stay in that XCD's L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+    constexpr int64_t kXcd = 8;
+    const int64_t q = nb / kXcd, r = nb % kXcd;
+    const int64_t x = b % kXcd, i = b / kXcd;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Tile prologue shared by the drivers: zeroes the block's LDS accumulator
+// range [colptr[j0], colptr[jend]) and runs the op's per-block set-up.
+struct TileRange {
+    int64_t tile_id, j0, jend;
+    int base, size;
+};
+template <class Op, class Acc>
+__device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* acc,
+                                                unsigned char* smem) {
+    TileRange t;
+    t.tile_id = xcd_tile(blockIdx.x, gridDim.x);
+    t.j0 = t.tile_id * kW;
+    t.jend = t.j0 + kW < tc.n ? t.j0 + kW : tc.n;
+    t.base = tc.colptr[t.j0];
+    t.size = tc.colptr[t.jend] - t.base;
+    op.block_init(smem + (size_t)op.acc_bytes_max);
+    for (int i = threadIdx.x; i < t.size; i += kW) acc[i] = Op::zero();
+    __syncthreads();
+    return t;
+}
+
+// Tile epilogue: the finished accumulator range leaves as contiguous stores.
+template <class Op, class Acc>
+__device__ __forceinline__ void tile_end(const Op& op, const TileRange& t, const Acc* acc) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < t.size; i += kW) op.store(t.base + i, acc[i]);
+}
+
+}  // namespace csc
+}  // namespace mpk

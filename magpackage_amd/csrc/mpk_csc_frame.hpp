@@ -1,0 +1,483 @@
+// Closed-form CSC assembly (FP64 / FP32_LOCAL modes): element arithmetic in a
+// node frame (lo, hi, o2, o3) whose edge (0,1) is the column's edge, and the
+// two drivers that feed it.
+//
+//   frame driver: the column's incidences in element order; per incidence a
+//                 16-byte record (element, local edge, orientation bits, row
+//                 positions), the element's node ids T[e] and its two
+//                 opposite nodes are gathered.
+//   ring driver:  the incidences in rotation order around the column edge,
+//                 so consecutive elements share a face: each record carries
+//                 the element and the one new opposite node (24-bit ids), and
+//                 no T[e] gather is needed.  Used when every column is a
+//                 simple ring or chain (manifold meshes).
+//
+// Both keep the lo/hi endpoint coordinates of the column edge in registers
+// for the whole column and run a software pipeline over the incidences.
+// Summation order per CSC entry is the driver's incidence order (fixed, so
+// results are deterministic); the FP64 closed forms agree with the
+// reference-order arithmetic to <= 1e-12 relative.
+#pragma once
+
+#include "mpk_csc_common.hpp"
+
+namespace mpk {
+namespace csc {
+
+// In-frame row positions pos'_0..pos'_5 (bytes) of the 6 frame edges.
+struct FPos {
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ int fpos(const FPos& p, int rp) {
+    return rp < 4 ? (int)((p.lo >> (8 * rp)) & 0xffu) : (int)((p.hi >> (8 * (rp - 4))) & 0xffu);
+}
+__device__ __forceinline__ double fsign(uint32_t fbits, int rp) {
+    return ((fbits >> rp) & 1u) ? 1.0 : -1.0;
+}
+
+// Frame record: x = element, y = c | f << 8 | pos'0 << 16 | pos'1 << 24,
+// z = pos'2..pos'5.
+__device__ __forceinline__ uint32_t frec_elem(uint4 r) { return r.x; }
+__device__ __forceinline__ int frec_c(uint4 r) { return (int)(r.y & 0xffu); }
+__device__ __forceinline__ uint32_t frec_f(uint4 r) { return (r.y >> 8) & 0x3fu; }
+__device__ __forceinline__ FPos frec_pos(uint4 r) { return FPos{(r.y >> 16) | (r.z << 16), r.z >> 16}; }
+
+// Ring record: x = element | f << 24, y = o_new | pos'0 << 24,
+// z = pos'1..pos'4, w = pos'5 | o_start << 8.
+__device__ __forceinline__ uint32_t rrec_elem(uint4 r) { return r.x & 0xffffffu; }
+__device__ __forceinline__ uint32_t rrec_f(uint4 r) { return (r.x >> 24) & 0x3fu; }
+__device__ __forceinline__ int rrec_new(uint4 r) { return (int)(r.y & 0xffffffu); }
+__device__ __forceinline__ int rrec_start(uint4 r) { return (int)(r.w >> 8); }
+__device__ __forceinline__ FPos rrec_pos(uint4 r) {
+    return FPos{(r.y >> 24) | (r.z << 8), (r.z >> 24) | ((r.w & 0xffu) << 8)};
+}
+
+__device__ __forceinline__ Vec3 ld_node(const double4* __restrict__ P4, int node) {
+    const double2 xy = *reinterpret_cast<const double2*>(&P4[node].x);
+    const double z = P4[node].z;
+    return Vec3{xy.x, xy.y, z};
+}
+
+struct FrameNodes {
+    Vec3 x[4];
+};
+
+// Geometry of one incidence in the frame.
+struct FrameGeom {
+    Vec3 u[6];   // opposite-edge vectors: J c_r' = 2 u_r'
+    Vec3 X[4];   // det * grad lambda_i (rows of adj(J))
+    double det;  // signed determinant in the frame
+};
+
+__device__ __forceinline__ FrameGeom frame_geom(const FrameNodes& nf, bool need_x) {
+    FrameGeom g;
+    const Vec3 x0 = nf.x[0], x1 = nf.x[1], x2 = nf.x[2], x3 = nf.x[3];
+    const Vec3 d1 = sub(x1, x0), d2 = sub(x2, x0), d3 = sub(x3, x0);
+    g.u[0] = sub(x3, x2);
+    g.u[1] = sub(x1, x3);
+    g.u[2] = sub(x2, x1);
+    g.u[3] = d3;
+    g.u[4] = Vec3{-d2.x, -d2.y, -d2.z};
+    g.u[5] = d1;
+    const Vec3 X1 = cross(d2, d3);
+    g.det = dot(d1, X1);
+    if (need_x) {
+        const Vec3 X2 = cross(d3, d1), X3 = cross(d1, d2);
+        g.X[1] = X1;
+        g.X[2] = X2;
+        g.X[3] = X3;
+        g.X[0] = Vec3{-(X1.x + X2.x + X3.x), -(X1.y + X2.y + X3.y), -(X1.z + X2.z + X3.z)};
+    }
+    return g;
+}
+
+// Mass column 0 in the frame (without sigma/|det| and signs): the rule's sum
+// of W_r' . W_0 = Q_ak G_bl - Q_al G_bk - Q_bk G_al + Q_bl G_ak for
+// r' = (a,b), (k,l) = (0,1), with symmetric moments qd / qo.
+template <class R, class V>
+__device__ __forceinline__ void mass_col_t(const V (&X)[4], R qd, R qo, R (&m)[6]) {
+    auto d = [](const V& a, const V& b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    const R H00 = d(X[0], X[0]), H01 = d(X[0], X[1]), H11 = d(X[1], X[1]);
+    const R H20 = d(X[2], X[0]), H21 = d(X[2], X[1]), H30 = d(X[3], X[0]), H31 = d(X[3], X[1]);
+    m[0] = qd * (H11 + H00) - (qo + qo) * H01;
+    m[1] = qd * H21 - qo * H20 - qo * H01 + qo * H00;
+    m[2] = qd * H31 - qo * H30 - qo * H01 + qo * H00;
+    m[3] = qo * H21 - qd * H20 - qo * H11 + qo * H01;
+    m[4] = qo * H31 - qd * H30 - qo * H11 + qo * H01;
+    m[5] = qo * (H31 - H30 - H21 + H20);
+}
+
+template <int kPrec>
+__device__ __forceinline__ void mass_col(const FrameGeom& g, const CscQuad& q, double (&m)[6]) {
+    if (kPrec == MPK_PREC_FP32_LOCAL) {
+        const Vec3f X[4] = {tof(g.X[0]), tof(g.X[1]), tof(g.X[2]), tof(g.X[3])};
+        float mf[6];
+        mass_col_t<float, Vec3f>(X, (float)q.qd, (float)q.qo, mf);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) m[r] = (double)mf[r];
+    } else {
+        mass_col_t<double, Vec3>(g.X, q.qd, q.qo, m);
+    }
+}
+
+// Column 0 of the curl-curl Gram (u_r' . u_0) in FP64 or FP32.
+template <int kPrec>
+__device__ __forceinline__ double curl_dot(const FrameGeom& g, int rp) {
+    if (kPrec == MPK_PREC_FP32_LOCAL) return (double)dotf(tof(g.u[rp]), tof(g.u[0]));
+    return dot(g.u[rp], g.u[0]);
+}
+
+struct NoIds {};
+
+// ------------------------------------------------------------------ ops --
+// Each op: Acc (LDS accumulator type), Elem (per-element data gathered one
+// stage ahead), Ids (per-incidence gathers two stages ahead), Col (per-column
+// state), and compute() on the frame nodes.
+
+template <int kPrec>
+struct FrameCurl {
+    using Acc = double;
+    using Ids = NoIds;
+    using Elem = double;  // nu
+    struct Col {};
+    const double* __restrict__ nu;
+    double* __restrict__ out;
+    CscQuad qa;
+    int acc_bytes_max;
+    __device__ static Acc zero() { return 0.0; }
+    __device__ void block_init(unsigned char*) const {}
+    __device__ Col col_begin() const { return Col{}; }
+    __device__ void col_end(int64_t, Col&) const {}
+    __device__ void store(int64_t p, Acc a) const { out[p] = a; }
+    __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
+    __device__ Elem elem(uint32_t e, const Ids&) const { return nu[e]; }
+    __device__ void compute(const FrameNodes& nodes, const Elem& nuv, uint32_t f, const FPos& pos,
+                            Acc* acc, Col&) const {
+        const FrameGeom g = frame_geom(nodes, false);
+        const double scale = 4.0 * qa.wsum * nuv / fabs(g.det) * fsign(f, 0);
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const int p = fpos(pos, rp);
+            const double d = curl_dot<kPrec>(g, rp);
+            if (p != 0xff) acc[p] += d * (scale * fsign(f, rp));
+        }
+    }
+};
+
+template <int kPrec>
+struct FrameMass {
+    using Acc = double;
+    using Ids = NoIds;
+    using Elem = double;  // sigma
+    struct Col {};
+    const double* __restrict__ coeffs;  // NULL: sigma = 1
+    double* __restrict__ out;
+    CscQuad qa;
+    int acc_bytes_max;
+    __device__ static Acc zero() { return 0.0; }
+    __device__ void block_init(unsigned char*) const {}
+    __device__ Col col_begin() const { return Col{}; }
+    __device__ void col_end(int64_t, Col&) const {}
+    __device__ void store(int64_t p, Acc a) const { out[p] = a; }
+    __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
+    __device__ Elem elem(uint32_t e, const Ids&) const { return coeffs ? coeffs[e] : 1.0; }
+    __device__ void compute(const FrameNodes& nodes, const Elem& sigma, uint32_t f,
+                            const FPos& pos, Acc* acc, Col&) const {
+        if (!(fabs(sigma) > 1e-12)) return;  // zero block (mass_kernel_mex.cpp:69)
+        const FrameGeom g = frame_geom(nodes, true);
+        const double scale = sigma / fabs(g.det) * fsign(f, 0);
+        double m[6];
+        mass_col<kPrec>(g, qa, m);
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const int p = fpos(pos, rp);
+            if (p != 0xff) acc[p] += m[rp] * (scale * fsign(f, rp));
+        }
+    }
+};
+
+template <int kPrec>
+struct FrameKM {
+    using Acc = double2;
+    using Ids = NoIds;
+    using Elem = double2;  // (nu, sigma)
+    struct Col {};
+    const double* __restrict__ nu;
+    const double* __restrict__ sigma;
+    double2* __restrict__ out;
+    double omega;
+    CscQuad qk, qm;
+    int acc_bytes_max;
+    __device__ static Acc zero() { return make_double2(0.0, 0.0); }
+    __device__ void block_init(unsigned char*) const {}
+    __device__ Col col_begin() const { return Col{}; }
+    __device__ void col_end(int64_t, Col&) const {}
+    __device__ void store(int64_t p, Acc a) const {
+        // K + 1j*omega*M: real part K, imaginary part omega * (sum of M)
+        out[p] = make_double2(a.x, omega * a.y);
+    }
+    __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
+    __device__ Elem elem(uint32_t e, const Ids&) const { return make_double2(nu[e], sigma[e]); }
+    __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
+                            Acc* acc, Col&) const {
+        const bool has_m = fabs(el.y) > 1e-12;
+        const FrameGeom g = frame_geom(nodes, has_m);
+        const double inv = 1.0 / fabs(g.det);
+        const double f0 = fsign(f, 0);
+        const double ks = 4.0 * qk.wsum * el.x * inv * f0;
+        const double ms = el.y * inv * f0;
+        double m[6] = {0, 0, 0, 0, 0, 0};
+        if (has_m) mass_col<kPrec>(g, qm, m);
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const int p = fpos(pos, rp);
+            const double d = curl_dot<kPrec>(g, rp);
+            const double s = fsign(f, rp);
+            if (p != 0xff) {
+                Acc a = acc[p];
+                a.x += d * (ks * s);
+                if (has_m) a.y += m[rp] * (ms * s);
+                acc[p] = a;
+            }
+        }
+    }
+};
+
+// Newton Jacobian + residual.  The element's 6 edge DOFs are the column's own
+// row indices (rowind[colptr[j] + pos'_r'] == dof - 1), gathered two stages
+// ahead; the solution values of those DOFs one stage ahead.
+struct FrameRows {
+    int row[6];  // 0-based DOF of frame edge r', -1 when inactive
+};
+struct JacElem {
+    double a[6];  // SolA of the frame edges (0 when inactive)
+    int tag;
+};
+
+template <int kPrec, bool kCalcJ>
+struct FrameJac {
+    using Acc = double;
+    using Ids = FrameRows;
+    using Elem = JacElem;
+    struct Col {
+        double r;
+    };
+    const int32_t* __restrict__ tags;
+    const double* __restrict__ sol;
+    MaterialView mat;
+    int ntags, nbrk, ncoe;
+    double* __restrict__ out;
+    double* __restrict__ res;
+    CscQuad qa;
+    int acc_bytes_max;
+    const double* brk_s;  // set per block (LDS or HBM)
+    const double* coe_s;
+    __device__ static Acc zero() { return 0.0; }
+    __device__ void block_init(unsigned char* lds) {
+        if (nbrk + ncoe <= kMatLds) {
+            double* tab = reinterpret_cast<double*>(lds);
+            for (int i = threadIdx.x; i < nbrk; i += kW) tab[i] = mat.brk[i];
+            for (int i = threadIdx.x; i < ncoe; i += kW) tab[nbrk + i] = mat.coe[i];
+            brk_s = tab;
+            coe_s = tab + nbrk;
+        } else {
+            brk_s = mat.brk;
+            coe_s = mat.coe;
+        }
+    }
+    __device__ Col col_begin() const { return Col{0.0}; }
+    __device__ void col_end(int64_t j, Col& cs) const { res[j] = cs.r; }
+    __device__ void store(int64_t p, Acc a) const {
+        if (kCalcJ) out[p] = a;
+    }
+    __device__ Ids ids(const TileCtx& tc, int cb, const FPos& pos) const {
+        Ids d;
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const int p = fpos(pos, rp);
+            d.row[rp] = (p != 0xff) ? tc.rowind[cb + p] : -1;
+        }
+        return d;
+    }
+    __device__ Elem elem(uint32_t e, const Ids& d) const {
+        Elem el;
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) el.a[rp] = d.row[rp] >= 0 ? sol[d.row[rp]] : 0.0;
+        const int tag = tags[e];
+        // tags are validated by the host layer; clamp so a bad tag cannot read
+        // outside the material table
+        el.tag = tag < 1 ? 1 : (tag > ntags ? ntags : tag);
+        return el;
+    }
+    __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
+                            Acc* acc, Col& cs) const {
+        const FrameGeom g = frame_geom(nodes, false);
+        // C'_r' = 2 u_r' / det;  B = sum a'_r' C'_r' with a' = A * f
+        const double two_inv = 2.0 / g.det;
+        Vec3 Bv{0.0, 0.0, 0.0};
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const double ap = el.a[rp] * fsign(f, rp);
+            Bv.x += ap * g.u[rp].x;
+            Bv.y += ap * g.u[rp].y;
+            Bv.z += ap * g.u[rp].z;
+        }
+        Bv.x *= two_inv;
+        Bv.y *= two_inv;
+        Bv.z *= two_inv;
+        double nu, dnu;
+        eval_nu(dot(Bv, Bv), el.tag, mat, brk_s, coe_s, nu, dnu);
+        // w|det| summed over the rule; C'.C' = 4 u.u / det^2
+        const double wd = qa.wsum * fabs(g.det);
+        const double f0 = fsign(f, 0);
+        const double BC0 = two_inv * dot(Bv, g.u[0]);
+        cs.r += nu * BC0 * wd * f0;
+        if (kCalcJ) {
+            const double cc = 4.0 / (g.det * g.det);
+#pragma unroll
+            for (int rp = 0; rp < 6; ++rp) {
+                const int p = fpos(pos, rp);
+                const double BCr = two_inv * dot(Bv, g.u[rp]);
+                const double uu = curl_dot<kPrec>(g, rp);
+                const double val = nu * (cc * uu) + 2.0 * dnu * BCr * BC0;
+                if (p != 0xff) acc[p] += val * wd * (fsign(f, rp) * f0);
+            }
+        }
+    }
+};
+
+// -------------------------------------------------------------- drivers --
+// Frame driver, element order.  Per lane, a 4-stage pipeline:
+//   stage 3: frame record                            for k + 3
+//   stage 2: T[e] and the op's Ids                   for k + 2
+//   stage 1: the 2 opposite nodes and Elem           for k + 1
+//   stage 0: arithmetic and LDS accumulation         for k
+template <class Op>
+__global__ void __launch_bounds__(kW) k_csc_frame(TileCtx tc, Op op) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    using Acc = typename Op::Acc;
+    using Ids = typename Op::Ids;
+    using Elem = typename Op::Elem;
+    Acc* acc = reinterpret_cast<Acc*>(smem);
+    const TileRange tr = tile_begin(tc, op, acc, smem);
+    const int lane = threadIdx.x;
+    const int64_t j = tr.j0 + lane;
+    if (j < tr.jend) {
+        const int cbase = tc.colptr[j];
+        Acc* col = acc + (cbase - tr.base);
+        typename Op::Col cs = op.col_begin();
+        const int cnt = tc.inc_ptr[j + 1] - tc.inc_ptr[j];
+        if (cnt > 0) {
+            // k-major records: incidence k of this column at rec[k * kW]
+            const uint4* __restrict__ rec = tc.inc_frame + tc.frame_base[tr.tile_id] + lane;
+            const uint4 z4 = make_uint4(0, 0, 0, 0);
+            uint4 r0 = rec[0], r1 = z4, r2 = z4;
+            if (1 < cnt) r1 = rec[kW];
+            if (2 < cnt) r2 = rec[2 * kW];
+            int4 t0 = tc.T[frec_elem(r0)], t1 = t0;
+            Ids d0 = op.ids(tc, cbase, frec_pos(r0)), d1 = d0;
+            if (1 < cnt) {
+                t1 = tc.T[frec_elem(r1)];
+                d1 = op.ids(tc, cbase, frec_pos(r1));
+            }
+            // the column edge's endpoints, common to all its incidences
+            const int c0 = frec_c(r0);
+            const int na = sel6(c0, t0.x, t0.x, t0.x, t0.y, t0.y, t0.z);
+            const int nb = sel6(c0, t0.y, t0.z, t0.w, t0.z, t0.w, t0.w);
+            const Vec3 xlo = ld_node(tc.P4, na <= nb ? na : nb);
+            const Vec3 xhi = ld_node(tc.P4, na <= nb ? nb : na);
+            auto opp2 = [&](int4 t, int c) {
+                return ld_node(tc.P4, sel6(c, t.z, t.y, t.y, t.x, t.x, t.x));
+            };
+            auto opp3 = [&](int4 t, int c) {
+                return ld_node(tc.P4, sel6(c, t.w, t.w, t.z, t.w, t.z, t.y));
+            };
+            Vec3 a2 = opp2(t0, c0), a3 = opp3(t0, c0);
+            Elem e0 = op.elem(frec_elem(r0), d0);
+            for (int k = 0; k < cnt; ++k) {
+                Vec3 b2 = a2, b3 = a3;
+                Elem e1 = e0;
+                if (k + 1 < cnt) {
+                    b2 = opp2(t1, frec_c(r1));
+                    b3 = opp3(t1, frec_c(r1));
+                    e1 = op.elem(frec_elem(r1), d1);
+                }
+                int4 t2 = t1;
+                Ids d2 = d1;
+                if (k + 2 < cnt) {
+                    t2 = tc.T[frec_elem(r2)];
+                    d2 = op.ids(tc, cbase, frec_pos(r2));
+                }
+                uint4 r3 = z4;
+                if (k + 3 < cnt) r3 = rec[(k + 3) * kW];
+                op.compute(FrameNodes{{xlo, xhi, a2, a3}}, e0, frec_f(r0), frec_pos(r0), col, cs);
+                a2 = b2; a3 = b3; e0 = e1;
+                r0 = r1; r1 = r2; r2 = r3;
+                t1 = t2; d1 = d2;
+            }
+        }
+        op.col_end(j, cs);
+    }
+    tile_end(op, tr, acc);
+}
+
+// Ring driver, rotation order.  Per lane, a 3-stage pipeline:
+//   stage 2: ring record and the op's Ids            for k + 2
+//   stage 1: the new opposite node and Elem          for k + 1
+//   stage 0: arithmetic and LDS accumulation         for k
+template <class Op>
+__global__ void __launch_bounds__(kW) k_csc_ring(TileCtx tc, const int2* __restrict__ ends, Op op) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    using Acc = typename Op::Acc;
+    using Ids = typename Op::Ids;
+    using Elem = typename Op::Elem;
+    Acc* acc = reinterpret_cast<Acc*>(smem);
+    const TileRange tr = tile_begin(tc, op, acc, smem);
+    const int lane = threadIdx.x;
+    const int64_t j = tr.j0 + lane;
+    if (j < tr.jend) {
+        const int cbase = tc.colptr[j];
+        Acc* col = acc + (cbase - tr.base);
+        typename Op::Col cs = op.col_begin();
+        const int cnt = tc.inc_ptr[j + 1] - tc.inc_ptr[j];
+        if (cnt > 0) {
+            const uint4* __restrict__ rec = tc.inc_ring + tc.frame_base[tr.tile_id] + lane;
+            const uint4 z4 = make_uint4(0, 0, 0, 0);
+            const int2 lh = ends[j];
+            uint4 r0 = rec[0], r1 = z4;
+            if (1 < cnt) r1 = rec[kW];
+            Ids d0 = op.ids(tc, cbase, rrec_pos(r0)), d1 = d0;
+            if (1 < cnt) d1 = op.ids(tc, cbase, rrec_pos(r1));
+            const Vec3 xlo = ld_node(tc.P4, lh.x), xhi = ld_node(tc.P4, lh.y);
+            Vec3 xprev = ld_node(tc.P4, rrec_start(r0));
+            Vec3 xnew = ld_node(tc.P4, rrec_new(r0));
+            Elem e0 = op.elem(rrec_elem(r0), d0);
+            for (int k = 0; k < cnt; ++k) {
+                Vec3 ynew = xnew;
+                Elem e1 = e0;
+                if (k + 1 < cnt) {
+                    ynew = ld_node(tc.P4, rrec_new(r1));
+                    e1 = op.elem(rrec_elem(r1), d1);
+                }
+                uint4 r2 = z4;
+                Ids d2 = d1;
+                if (k + 2 < cnt) {
+                    r2 = rec[(k + 2) * kW];
+                    d2 = op.ids(tc, cbase, rrec_pos(r2));
+                }
+                op.compute(FrameNodes{{xlo, xhi, xprev, xnew}}, e0, rrec_f(r0), rrec_pos(r0), col,
+                           cs);
+                xprev = xnew;
+                xnew = ynew;
+                e0 = e1;
+                r0 = r1; r1 = r2;
+                d1 = d2;
+            }
+        }
+        op.col_end(j, cs);
+    }
+    tile_end(op, tr, acc);
+}
+
+}  // namespace csc
+}  // namespace mpk

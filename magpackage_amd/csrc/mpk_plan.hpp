@@ -41,6 +41,19 @@ struct mpk_plan {
     uint4* inc_frame = nullptr;
     int32_t* frame_base = nullptr;  // [ntiles+1]
     mpk::DevBuf buf_incframe, buf_framebase;
+
+    // Ring records (closed-form modes, meshes with < 2^24 elements and nodes):
+    // the incidences of each column in rotation order around the column edge,
+    // so consecutive elements share a face and each record brings one new
+    // node.  Same k-major tile layout and frame_base as inc_frame.  Frame of
+    // record k: (lo, hi, o_prev, o_new), o_prev = o_new of record k-1 (record
+    // 0: o_start).  Packing:
+    //   x = element (24 bits) | f bits << 24      y = o_new (24 bits) | pos'_0 << 24
+    //   z = pos'_1..pos'_4 (bytes)                w = pos'_5 | o_start << 8
+    bool ring = false;  // every column is a simple ring or chain
+    uint4* inc_ring = nullptr;
+    int2* col_ends = nullptr;  // [n] (lo, hi) global node ids of each column edge
+    mpk::DevBuf buf_incring, buf_colends;
     // Node coordinates padded to 32 bytes (x, y, z, 0), refreshed by every call
     // of a frame kernel from the caller's P.
     double4* P4 = nullptr;

@@ -365,6 +365,111 @@ __global__ void k_plan_frame(const int32_t* __restrict__ inc_ptr,
     }
 }
 
+// Ring records (layout in mpk_plan.hpp), one thread per column.  The column
+// edge (lo, hi) is shared by all its elements; each element adds two
+// opposite nodes.  For a manifold mesh the elements around an edge form a
+// closed ring (interior edge: every opposite node seen twice) or an open
+// chain (boundary edge: the two chain ends seen once); the walk starts at a
+// chain end (or at the first element's first opposite node) and follows the
+// shared opposite nodes.  Anything else sets err.
+constexpr int kRingMax = 32;
+
+__global__ void k_plan_ring(const int32_t* __restrict__ inc_ptr,
+                            const uint32_t* __restrict__ slot, const uint2* __restrict__ pos,
+                            const int4* __restrict__ T, const int32_t* __restrict__ frame_base,
+                            int64_t n, uint4* __restrict__ rec, int2* __restrict__ ends,
+                            int* __restrict__ err) {
+    const int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (j >= n) return;
+    const int kb = inc_ptr[j], d = inc_ptr[j + 1] - kb;
+    ends[j] = make_int2(0, 0);
+    if (d == 0) return;
+    {
+        const uint32_t s = slot[kb];
+        const int c = (int)(s % 6u);
+        const int4 t = T[s / 6u];
+        const int nd[4] = {t.x, t.y, t.z, t.w};
+        const int na = nd[edge_a(c)], nb = nd[edge_b(c)];
+        ends[j] = na <= nb ? make_int2(na, nb) : make_int2(nb, na);
+    }
+    if (d > kRingMax) {
+        atomicOr(err, 1);
+        return;
+    }
+    int o1[kRingMax], o2[kRingMax];
+    bool used[kRingMax];
+    for (int i = 0; i < d; ++i) {
+        const uint32_t s = slot[kb + i];
+        const int c = (int)(s % 6u);
+        const int4 t = T[s / 6u];
+        const int nd[4] = {t.x, t.y, t.z, t.w};
+        o1[i] = nd[perm_node(c, 2)];
+        o2[i] = nd[perm_node(c, 3)];
+        used[i] = false;
+    }
+    auto count = [&](int v) {
+        int m = 0;
+        for (int i = 0; i < d; ++i) m += (o1[i] == v) + (o2[i] == v);
+        return m;
+    };
+    int start = o1[0];
+    for (int i = 0; i < d; ++i) {
+        const int c1 = count(o1[i]), c2 = count(o2[i]);
+        if (c1 > 2 || c2 > 2) {
+            atomicOr(err, 1);
+            return;
+        }
+        if (c1 == 1) { start = o1[i]; break; }
+        if (c2 == 1) { start = o2[i]; break; }
+    }
+    const int64_t base = frame_base[j / kTileCols] + (j % kTileCols);
+    int cur = start;
+    for (int k = 0; k < d; ++k) {
+        int i = -1;
+        for (int q = 0; q < d; ++q)
+            if (!used[q] && (o1[q] == cur || o2[q] == cur)) {
+                i = q;
+                break;
+            }
+        if (i < 0) {
+            atomicOr(err, 1);
+            return;
+        }
+        used[i] = true;
+        const int onew = (o1[i] == cur) ? o2[i] : o1[i];
+        const uint32_t s = slot[kb + i];
+        const uint32_t e = s / 6u;
+        const int c = (int)(s - 6u * e);
+        const uint2 pw = pos[kb + i];
+        const int4 t = T[e];
+        const int nd[4] = {t.x, t.y, t.z, t.w};
+        const int a = edge_a(c), b = edge_b(c);
+        // frame (lo, hi, o_prev, o_new) as local node indices
+        int fr[4];
+        fr[0] = nd[a] <= nd[b] ? a : b;
+        fr[1] = nd[a] <= nd[b] ? b : a;
+        fr[2] = perm_node(c, 2);
+        fr[3] = perm_node(c, 3);
+        if (nd[fr[2]] != cur) {
+            const int tmp = fr[2];
+            fr[2] = fr[3];
+            fr[3] = tmp;
+        }
+        uint32_t fmask = 0, p[6];
+        for (int rp = 0; rp < 6; ++rp) {
+            const int u = fr[edge_a(rp)], v = fr[edge_b(rp)];
+            if (nd[u] < nd[v]) fmask |= 1u << rp;
+            const int r = u < v ? edge_index(u, v) : edge_index(v, u);
+            p[rp] = r < 4 ? ((pw.x >> (8 * r)) & 0xffu) : ((pw.y >> (8 * (r - 4))) & 0xffu);
+        }
+        rec[base + (int64_t)k * kTileCols] =
+            make_uint4(e | (fmask << 24), (uint32_t)onew | (p[0] << 24),
+                       p[1] | (p[2] << 8) | (p[3] << 16) | (p[4] << 24),
+                       p[5] | ((uint32_t)start << 8));
+        cur = onew;
+    }
+}
+
 __global__ void k_tile_max(const int32_t* __restrict__ colptr, int64_t n,
                            int* __restrict__ maxv) {
     const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
@@ -740,6 +845,29 @@ int mpk_plan_create(const int32_t* T, const int32_t* cell_dofs, int64_t ne, int6
         hipLaunchKernelGGL(k_plan_frame, dim3(grid_for(ndofs, kB)), dim3(kB), 0, st,
                            plan->inc_ptr, plan->inc_slot, plan->inc_pos, plan->T,
                            plan->frame_base, ndofs, plan->inc_frame);
+        // ring records: ids are packed in 24 bits
+        if (ne < (int64_t(1) << 24) && nn < (int64_t(1) << 24)) {
+            if (plan->buf_incring.alloc((size_t)nslots * 16) != hipSuccess ||
+                plan->buf_colends.alloc((size_t)ndofs * 8) != hipSuccess) {
+                set_error("hipMalloc failed for ring records");
+                return fail(MPK_ERR_HIP);
+            }
+            (void)hipMemsetAsync(err.p, 0, sizeof(int), st);
+            hipLaunchKernelGGL(k_plan_ring, dim3(grid_for(ndofs, kB)), dim3(kB), 0, st,
+                               plan->inc_ptr, plan->inc_slot, plan->inc_pos, plan->T,
+                               plan->frame_base, ndofs, plan->buf_incring.as<uint4>(),
+                               plan->buf_colends.as<int2>(), err.as<int>());
+            int hr = 0;
+            if (int rc = check_err(err.as<int>(), st, &hr)) return fail(rc);
+            plan->ring = (hr == 0);
+            if (plan->ring) {
+                plan->inc_ring = plan->buf_incring.as<uint4>();
+                plan->col_ends = plan->buf_colends.as<int2>();
+            } else {
+                plan->buf_incring.release();
+                plan->buf_colends.release();
+            }
+        }
     }
     if (plan->buf_P4.alloc((size_t)(nn > 0 ? nn : 1) * 32) != hipSuccess) {
         set_error("hipMalloc failed for padded coordinates");
@@ -768,6 +896,13 @@ int mpk_plan_info(const mpk_plan* plan, int64_t* nnz, int64_t* ncols, int64_t* n
     if (ne) *ne = plan->ne;
     if (n_incidences) *n_incidences = plan->ninc;
     if (max_tile_nnz) *max_tile_nnz = plan->max_tile_nnz;
+    return MPK_OK;
+}
+
+int mpk_plan_ring_order(const mpk_plan* plan, int* ring) {
+    clear_error();
+    MPK_REQUIRE(plan && ring, MPK_ERR_ARGS, "NULL argument");
+    *ring = plan->ring ? 1 : 0;
     return MPK_OK;
 }
 

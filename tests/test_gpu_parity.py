@@ -153,6 +153,18 @@ def cfg1(gpu):
     return mesh, M, dh, space, asm, mi
 
 
+@pytest.fixture(params=["ring", "element"])
+def driver(request, monkeypatch):
+    """Closed-form modes: rotation-order (default) and element-order drivers."""
+    monkeypatch.setenv("MPK_CSC_DRIVER", request.param)
+    return request.param
+
+
+def _one_exact_driver(prec, driver):
+    if prec == "fp64_exact" and driver != "ring":
+        pytest.skip("the exact mode has a single (element-order) driver")
+
+
 def test_plan_pattern_matches_sparse(cfg1):
     mesh, M, dh, space, asm, mi = cfg1
     plan = asm.plan(space)
@@ -161,12 +173,14 @@ def test_plan_pattern_matches_sparse(cfg1):
                           qw, curl)
     c, r, _ = O.sparse(I, J, V, mi["ndof"], mi["ndof"], drop_zeros=False)
     colptr, rowind = plan.pattern()
-    assert plan.nnz == r.size == 938060  # nnz(K) of the 20^3 Kuhn box    assert np.array_equal(colptr.cpu().numpy(), c)
+    assert plan.nnz == r.size == 938060  # nnz(K) of the 20^3 Kuhn box
+    assert plan.ring_order  # a Kuhn mesh is manifold: every column is a ring or chain    assert np.array_equal(colptr.cpu().numpy(), c)
     assert np.array_equal(rowind.cpu().numpy(), r)
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
-def test_csc_stiffness(cfg1, prec):
+def test_csc_stiffness(cfg1, prec, driver):
+    _one_exact_driver(prec, driver)
     mesh, M, dh, space, asm, mi = cfg1
     nu = config1_nu(mesh)
     qw, curl, _ = stiffness_tables()
@@ -183,7 +197,8 @@ def test_csc_stiffness(cfg1, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
-def test_csc_mass(cfg1, prec):
+def test_csc_mass(cfg1, prec, driver):
+    _one_exact_driver(prec, driver)
     mesh, M, dh, space, asm, mi = cfg1
     qm, val, _ = mass_tables(2)
     rng = np.random.default_rng(7)
@@ -209,7 +224,8 @@ def test_mass_weighted_empty_when_sigma_zero(cfg1):
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
-def test_csc_frequency_operator(cfg1, prec):
+def test_csc_frequency_operator(cfg1, prec, driver):
+    _one_exact_driver(prec, driver)
     mesh, M, dh, space, asm, mi = cfg1
     nu = config1_nu(mesh)
     sig = np.where(mesh.RegionTags == 2, 3.5e7, 0.0)
@@ -233,7 +249,8 @@ def test_csc_frequency_operator(cfg1, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
-def test_csc_jacobian(cfg1, prec):
+def test_csc_jacobian(cfg1, prec, driver):
+    _one_exact_driver(prec, driver)
     mesh, M, dh, space, asm, mi = cfg1
     lib = {1: create_linear(1.0), 2: marrocco()}
     packed = pack_material_data(lib)
