@@ -167,8 +167,16 @@ int mpk_plan_ring_order(const mpk_plan* plan, int* ring);
 /* Device pointers owned by the plan: colptr [n+1], rowind [nnz] (0-based). */
 int mpk_plan_pattern(const mpk_plan* plan, const int32_t** colptr, const int32_t** rowind);
 
+/* Loads node coordinates P [3 x nn] (device doubles) into the plan's padded
+ * copy read by the FP64 / FP32_LOCAL kernels.  Those kernels accept P == NULL
+ * and then use this copy: a Newton loop on a fixed mesh loads P once instead
+ * of once per assembly.  No reference counterpart (the MEX kernels take P on
+ * every call, src/Assembler.m:37-38); FP64_EXACT always needs P. */
+int mpk_plan_set_coords(mpk_plan* plan, const double* P, void* stream);
+
 /* CSC values of K (curl-curl, src/mex/assemble_curl_curl_kernel_mex.cpp:18-121
- * + sparse()).  P [3 x nn] device doubles, nu [ne] device.  vals [nnz]. */
+ * + sparse()).  P [3 x nn] device doubles (NULL: coordinates loaded by
+ * mpk_plan_set_coords, FP64 / FP32_LOCAL only), nu [ne] device.  vals [nnz]. */
 int mpk_csc_curl_curl(const mpk_plan* plan, const double* P, const double* nu,
                       const mpk_quadrature* quad, int precision, double* vals, void* stream);
 

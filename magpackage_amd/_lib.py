@@ -79,6 +79,7 @@ _SIGS = {
     "mpk_plan_info": ([_plan_p, _pi64, _pi64, _pi64, _pi64, _pi64], _i32),
     "mpk_plan_pattern": ([_plan_p, C.POINTER(_vp), C.POINTER(_vp)], _i32),
     "mpk_plan_ring_order": ([_plan_p, C.POINTER(C.c_int)], _i32),
+    "mpk_plan_set_coords": ([_plan_p, _vp, _vp], _i32),
     "mpk_csc_curl_curl": ([_plan_p, _vp, _vp, _pq, _i32, _vp, _vp], _i32),
     "mpk_csc_mass": ([_plan_p, _vp, _vp, _pq, _i32, _vp, _vp], _i32),
     "mpk_csc_curl_curl_mass_complex": ([_plan_p, _vp, _vp, _vp, _d, _pq, _pq, _i32, _vp, _vp],

@@ -93,8 +93,10 @@ int lds_bytes(const mpk_plan* plan, Op& op, size_t acc_size, size_t extra, size_
 }
 
 template <class Op>
-int launch_exact(const mpk_plan* plan, Op op, size_t acc_size, size_t extra, hipStream_t st) {
+int launch_exact(const mpk_plan* plan, const double* P, Op op, size_t acc_size, size_t extra,
+                 hipStream_t st) {
     if (plan->n == 0) return MPK_OK;
+    MPK_REQUIRE(P, MPK_ERR_ARGS, "FP64_EXACT mode needs the coordinate array P");
     size_t lds = 0;
     if (int rc = lds_bytes(plan, op, acc_size, extra, &lds)) return rc;
     hipLaunchKernelGGL(k_csc_exact<Op>, dim3(grid_for(plan->n, kW)), dim3(kW), lds, st,
@@ -103,27 +105,40 @@ int launch_exact(const mpk_plan* plan, Op op, size_t acc_size, size_t extra, hip
     return MPK_OK;
 }
 
-// Closed-form kernels read a 32-byte-padded coordinate copy: refresh it from
-// P first (stream-ordered, so a P changed between calls is picked up).  The
-// ring driver runs when the plan has ring records, unless the environment
-// sets MPK_CSC_DRIVER=element (element-order driver, for tests and timing).
+int load_coords(const mpk_plan* plan, const double* P, hipStream_t st) {
+    if (plan->nn > 0 && plan->P4) {
+        hipLaunchKernelGGL(k_pad_coords, dim3(grid_for(plan->nn, 256)), dim3(256), 0, st, P,
+                           plan->nn, plan->P4);
+        MPK_HIP_TRY(hipGetLastError());
+    }
+    plan->coords_loaded = true;
+    return MPK_OK;
+}
+
+// Closed-form kernels read the plan's 32-byte-padded coordinate copy.  With P
+// given it is refreshed from P first (stream-ordered, so a P changed between
+// calls is picked up); with P == NULL the copy made by the last
+// mpk_plan_set_coords (or the last call with P) is used.  The ring driver runs
+// when the plan has ring records, unless the environment sets
+// MPK_CSC_DRIVER=element (element-order driver, for tests and timing).
 template <class Op>
 int launch_frame(const mpk_plan* plan, const double* P, Op op, size_t acc_size, size_t extra,
                  hipStream_t st) {
     if (plan->n == 0) return MPK_OK;
     size_t lds = 0;
     if (int rc = lds_bytes(plan, op, acc_size, extra, &lds)) return rc;
-    if (plan->nn > 0 && plan->P4) {
-        hipLaunchKernelGGL(k_pad_coords, dim3(grid_for(plan->nn, 256)), dim3(256), 0, st, P,
-                           plan->nn, plan->P4);
-        MPK_HIP_TRY(hipGetLastError());
+    if (P) {
+        if (int rc = load_coords(plan, P, st)) return rc;
+    } else {
+        MPK_REQUIRE(plan->coords_loaded, MPK_ERR_ARGS,
+                    "P is NULL and the plan holds no coordinates (mpk_plan_set_coords)");
     }
     const char* drv = std::getenv("MPK_CSC_DRIVER");
     const bool element = drv && std::strcmp(drv, "element") == 0;
     const unsigned grid = grid_for(plan->n, kW);
     if (plan->ring && !element)
-        hipLaunchKernelGGL(k_csc_ring<Op>, dim3(grid), dim3(kW), lds, st, tile_ctx(plan),
-                           plan->col_ends, op);
+        hipLaunchKernelGGL((k_csc_ring_batch<Op, kRingBatch>), dim3(grid), dim3(kW), lds, st,
+                           tile_ctx(plan), plan->col_ends, op);
     else
         hipLaunchKernelGGL(k_csc_frame<Op>, dim3(grid), dim3(kW), lds, st, tile_ctx(plan), op);
     MPK_HIP_TRY(hipGetLastError());
@@ -138,11 +153,18 @@ using namespace mpk::csc;
 
 extern "C" {
 
+int mpk_plan_set_coords(mpk_plan* plan, const double* P, void* stream) {
+    clear_error();
+    MPK_REQUIRE(plan, MPK_ERR_ARGS, "NULL plan");
+    MPK_REQUIRE(P || plan->nn == 0, MPK_ERR_ARGS, "NULL coordinate array");
+    return load_coords(plan, P, as_stream(stream));
+}
+
 int mpk_csc_curl_curl(const mpk_plan* plan, const double* P, const double* nu,
                       const mpk_quadrature* quad, int precision, double* vals, void* stream) {
     clear_error();
     MPK_REQUIRE(plan, MPK_ERR_ARGS, "NULL plan");
-    MPK_REQUIRE(plan->nnz == 0 || (P && nu && vals), MPK_ERR_ARGS, "NULL array argument");
+    MPK_REQUIRE(plan->nnz == 0 || (nu && vals), MPK_ERR_ARGS, "NULL array argument");
     CscQuad qa;
     if (int rc = make_csc_quad(quad, true, qa)) return rc;
     if (precision != MPK_PREC_FP64_EXACT && !standard_curl_table(quad))
@@ -150,7 +172,7 @@ int mpk_csc_curl_curl(const mpk_plan* plan, const double* P, const double* nu,
     hipStream_t st = as_stream(stream);
     switch (precision) {
         case MPK_PREC_FP64_EXACT:
-            return launch_exact(plan, ExactCurl{P, nu, vals, qa, 0}, 8, 0, st);
+            return launch_exact(plan, P, ExactCurl{P, nu, vals, qa, 0}, 8, 0, st);
         case MPK_PREC_FP64:
             return launch_frame(plan, P, FrameCurl<MPK_PREC_FP64>{nu, vals, qa, 0}, 8, 0, st);
         case MPK_PREC_FP32_LOCAL:
@@ -164,13 +186,13 @@ int mpk_csc_mass(const mpk_plan* plan, const double* P, const double* coeffs,
                  const mpk_quadrature* quad, int precision, double* vals, void* stream) {
     clear_error();
     MPK_REQUIRE(plan, MPK_ERR_ARGS, "NULL plan");
-    MPK_REQUIRE(plan->nnz == 0 || (P && vals), MPK_ERR_ARGS, "NULL array argument");
+    MPK_REQUIRE(plan->nnz == 0 || vals, MPK_ERR_ARGS, "NULL array argument");
     CscQuad qa;
     if (int rc = make_csc_quad(quad, false, qa)) return rc;
     hipStream_t st = as_stream(stream);
     switch (precision) {
         case MPK_PREC_FP64_EXACT:
-            return launch_exact(plan, ExactMass{P, coeffs, vals, qa, 0}, 8, 0, st);
+            return launch_exact(plan, P, ExactMass{P, coeffs, vals, qa, 0}, 8, 0, st);
         case MPK_PREC_FP64:
             return launch_frame(plan, P, FrameMass<MPK_PREC_FP64>{coeffs, vals, qa, 0}, 8, 0, st);
         case MPK_PREC_FP32_LOCAL:
@@ -187,7 +209,7 @@ int mpk_csc_curl_curl_mass_complex(const mpk_plan* plan, const double* P, const 
                                    int precision, double* vals_c128, void* stream) {
     clear_error();
     MPK_REQUIRE(plan, MPK_ERR_ARGS, "NULL plan");
-    MPK_REQUIRE(plan->nnz == 0 || (P && nu && sigma && vals_c128), MPK_ERR_ARGS,
+    MPK_REQUIRE(plan->nnz == 0 || (nu && sigma && vals_c128), MPK_ERR_ARGS,
                 "NULL array argument");
     CscQuad qk, qm;
     if (int rc = make_csc_quad(quad_k, true, qk)) return rc;
@@ -198,7 +220,7 @@ int mpk_csc_curl_curl_mass_complex(const mpk_plan* plan, const double* P, const 
     double2* out = reinterpret_cast<double2*>(vals_c128);
     switch (precision) {
         case MPK_PREC_FP64_EXACT:
-            return launch_exact(plan, ExactKM{P, nu, sigma, out, omega, qk, qm, 0}, 16, 0, st);
+            return launch_exact(plan, P, ExactKM{P, nu, sigma, out, omega, qk, qm, 0}, 16, 0, st);
         case MPK_PREC_FP64:
             return launch_frame(plan, P, FrameKM<MPK_PREC_FP64>{nu, sigma, out, omega, qk, qm, 0},
                                 16, 0, st);
@@ -217,7 +239,7 @@ int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
                      double* j_vals, double* residual, void* stream) {
     clear_error();
     MPK_REQUIRE(plan && mat, MPK_ERR_ARGS, "NULL plan or material");
-    MPK_REQUIRE(plan->n == 0 || (P && tags && sol_a && residual), MPK_ERR_ARGS,
+    MPK_REQUIRE(plan->n == 0 || (tags && sol_a && residual), MPK_ERR_ARGS,
                 "NULL array argument");
     MPK_REQUIRE(!calc_j || plan->nnz == 0 || j_vals, MPK_ERR_ARGS, "NULL j_vals");
     CscQuad qa;
@@ -234,10 +256,10 @@ int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
     switch (precision) {
         case MPK_PREC_FP64_EXACT:
             if (calc_j)
-                return launch_exact(plan, ExactJac<true>{P, tags, sol_a, mv, ntags, nbrk, ncoe,
+                return launch_exact(plan, P, ExactJac<true>{P, tags, sol_a, mv, ntags, nbrk, ncoe,
                                                          j_vals, residual, qa, 0, nullptr, nullptr},
                                     8, extra, st);
-            return launch_exact(plan, ExactJac<false>{P, tags, sol_a, mv, ntags, nbrk, ncoe,
+            return launch_exact(plan, P, ExactJac<false>{P, tags, sol_a, mv, ntags, nbrk, ncoe,
                                                       j_vals, residual, qa, 0, nullptr, nullptr},
                                 8, extra, st);
         case MPK_PREC_FP64:

@@ -54,8 +54,9 @@ struct mpk_plan {
     uint4* inc_ring = nullptr;
     int2* col_ends = nullptr;  // [n] (lo, hi) global node ids of each column edge
     mpk::DevBuf buf_incring, buf_colends;
-    // Node coordinates padded to 32 bytes (x, y, z, 0), refreshed by every call
-    // of a frame kernel from the caller's P.
+    // Node coordinates padded to 32 bytes (x, y, z, 0), refreshed from the
+    // caller's P by mpk_plan_set_coords and by every frame-kernel call given P.
     double4* P4 = nullptr;
+    mutable bool coords_loaded = false;  // P4 holds the coordinates of some P
     mpk::DevBuf buf_P4;
 };

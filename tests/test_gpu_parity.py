@@ -174,7 +174,8 @@ def test_plan_pattern_matches_sparse(cfg1):
     c, r, _ = O.sparse(I, J, V, mi["ndof"], mi["ndof"], drop_zeros=False)
     colptr, rowind = plan.pattern()
     assert plan.nnz == r.size == 938060  # nnz(K) of the 20^3 Kuhn box
-    assert plan.ring_order  # a Kuhn mesh is manifold: every column is a ring or chain    assert np.array_equal(colptr.cpu().numpy(), c)
+    assert plan.ring_order  # a Kuhn mesh is manifold: every column is a ring or chain
+    assert np.array_equal(colptr.cpu().numpy(), c)
     assert np.array_equal(rowind.cpu().numpy(), r)
 
 
@@ -214,6 +215,36 @@ def test_csc_mass(cfg1, prec, driver):
             assert csc_equal_bits(Mg, ref)
         else:
             assert abs(Mg - ref).max() / abs(ref).max() < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
+def test_csc_preloaded_coordinates(cfg1, prec, driver):
+    """P == NULL reads the coordinates loaded by mpk_plan_set_coords: values
+    are bit-identical to the call given P; FP64_EXACT and a plan that never
+    saw coordinates reject P == NULL."""
+    import ctypes as C
+    import torch
+    from magpackage_amd import _lib as L
+    from magpackage_amd.assembler import Plan
+    mesh, M, dh, space, asm, mi = cfg1
+    nu = torch.from_numpy(config1_nu(mesh)).cuda()
+    q = asm.stiffness_quadrature()
+    pr = L.PRECISIONS[prec]
+    fresh = Plan(M, dh.DofMaps[space.toString()], dh.NumGlobalDofs)
+    v1 = torch.empty(fresh.nnz, dtype=torch.float64, device="cuda")
+    with pytest.raises(L.MpkError, match="no coordinates"):
+        L.check(L.lib.mpk_csc_curl_curl(fresh.handle, None, L.ptr(nu), C.byref(q), pr,
+                                        L.ptr(v1), L.stream_ptr()))
+    L.check(L.lib.mpk_csc_curl_curl(fresh.handle, L.ptr(M.P_dev), L.ptr(nu), C.byref(q), pr,
+                                    L.ptr(v1), L.stream_ptr()))
+    L.check(L.lib.mpk_plan_set_coords(fresh.handle, L.ptr(M.P_dev), L.stream_ptr()))
+    v2 = torch.full_like(v1, float("nan"))
+    L.check(L.lib.mpk_csc_curl_curl(fresh.handle, None, L.ptr(nu), C.byref(q), pr, L.ptr(v2),
+                                    L.stream_ptr()))
+    assert torch.equal(v1.view(torch.int64), v2.view(torch.int64))
+    with pytest.raises(L.MpkError, match="needs the coordinate array"):
+        L.check(L.lib.mpk_csc_curl_curl(fresh.handle, None, L.ptr(nu), C.byref(q),
+                                        L.PREC_FP64_EXACT, L.ptr(v2), L.stream_ptr()))
 
 
 def test_mass_weighted_empty_when_sigma_zero(cfg1):
