@@ -111,6 +111,23 @@ int mpk_jacobian_coo(const double* P, int64_t nn, const double* T, int64_t ne,
                      const mpk_material* mat, int calc_j, double* I_out, double* J_out,
                      double* V_out, double* R_idx, double* R_val, void* stream);
 
+/* Replaces assemble_source_kernel_mex (src/mex/assemble_source_kernel_mex.cpp:16-120).
+ * Js [3 x ne] device current density per element; quad carries Qw and
+ * ValRef.  Outputs 6*ne pairs, slot 6*e + i = (dofs[i], F_i * s[i]).
+ * Synchronises `stream` for the node-index check. */
+int mpk_source_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                   const double* cell_dofs, const double* signs, const double* Js,
+                   const mpk_quadrature* quad, double* I_out, double* V_out, void* stream);
+
+/* PostProcessor.computeElementB (src/post/PostProcessor.m:64-126) for one
+ * solution column: B_out [3 x ne] device, B_e = curl A at the element.  T
+ * [ne x 4] and cell_dofs [ne x 6] device int32, 1-based (0 = inactive DOF);
+ * signs from the node ids (Mesh.T2E_Sign); quad->ref_curl row q = 0 is the
+ * (constant) reference curl table.  Synchronises `stream` for index checks. */
+int mpk_element_b(const double* P, int64_t nn, const int32_t* T, const int32_t* cell_dofs,
+                  int64_t ne, const double* sol, int64_t num_sol, const mpk_quadrature* quad,
+                  double* B_out, void* stream);
+
 /* ========================================================================== */
 /* 2. COO -> CSC with MATLAB sparse() semantics                               */
 /* ========================================================================== */
@@ -202,6 +219,30 @@ int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
                      const double* sol_a, const mpk_material* mat,
                      const mpk_quadrature* quad, int calc_j, int precision,
                      double* j_vals, double* residual, void* stream);
+
+/* Fused Newton set in one pass over the plan (no single reference call: the
+ * reference assembles these with three MEX calls + sparse(),
+ * src/Assembler.m:26-76,182-231):
+ *   k_vals [nnz]  secant stiffness K(nu_e), nu_e = nu(|B_e|^2) from the B-H
+ *                 table at the element's B = curl of sol_a (the matrix
+ *                 assembleStiffness builds from those per-element nu);
+ *   m_vals [nnz]  sigma-mass, sigma [ne] device or NULL (sigma = 1);
+ *   j_vals [nnz]  Jacobian, as mpk_csc_jacobian;
+ *   residual [ndofs] R = K(nu) a, as mpk_csc_jacobian.
+ * Any output may be NULL.  FP64 / FP32_LOCAL only (MPK_ERR_ARGS for
+ * FP64_EXACT); P may be NULL after mpk_plan_set_coords. */
+int mpk_csc_newton_set(const mpk_plan* plan, const double* P, const int32_t* tags,
+                       const double* sol_a, const mpk_material* mat, const double* sigma,
+                       const mpk_quadrature* quad_k, const mpk_quadrature* quad_m,
+                       int precision, double* k_vals, double* m_vals, double* j_vals,
+                       double* residual, void* stream);
+
+/* Source vector F [ndofs] = sparse(I(valid), 1, V(valid), N, 1) of
+ * assemble_source_kernel_mex (src/Assembler.m:78-93), summed per DOF in
+ * triplet order (bit-identical); one lane per DOF, no atomics.  Js [3 x ne]
+ * device; quad: Qw + ValRef. */
+int mpk_csc_source(const mpk_plan* plan, const double* P, const double* Js,
+                   const mpk_quadrature* quad, double* F, void* stream);
 
 /* Emulates sparse()'s value-dependent rule on an assembled structural CSC:
  * copies the entries that are not exactly zero into (colptr_out [n+1] int64,

@@ -70,6 +70,9 @@ _SIGS = {
     "mpk_mass_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _pq, _vp, _vp, _vp, _vp], _i32),
     "mpk_jacobian_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _pq, _pm, _i32,
                           _vp, _vp, _vp, _vp, _vp, _vp], _i32),
+    "mpk_source_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _pq, _vp, _vp, _vp], _i32),
+    "mpk_element_b": ([_vp, _i64, _vp, _vp, _i64, _vp, _i64, _pq, _vp, _vp], _i32),
+    "mpk_csc_source": ([_plan_p, _vp, _vp, _pq, _vp, _vp], _i32),
     "mpk_sparse_coo_to_csc": ([_vp, _vp, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _i64,
                                _pi64, _vp], _i32),
     "mpk_generate_edges": ([_vp, _i64, _i64, _vp, _vp, _vp, _pi64, _vp], _i32),
@@ -85,6 +88,8 @@ _SIGS = {
     "mpk_csc_curl_curl_mass_complex": ([_plan_p, _vp, _vp, _vp, _d, _pq, _pq, _i32, _vp, _vp],
                                        _i32),
     "mpk_csc_jacobian": ([_plan_p, _vp, _vp, _vp, _pm, _pq, _i32, _i32, _vp, _vp, _vp], _i32),
+    "mpk_csc_newton_set": ([_plan_p, _vp, _vp, _vp, _pm, _vp, _pq, _pq, _i32, _vp, _vp, _vp,
+                            _vp, _vp], _i32),
     "mpk_csc_drop_zeros": ([_plan_p, _vp, _i32, _vp, _vp, _vp, _pi64, _vp], _i32),
     "mpk_spmv_sym": ([_vp, _vp, _vp, _i64, _vp, _vp, _vp], _i32),
     "mpk_csc_combine": ([_i64, _d, _vp, _d, _vp, _d, _vp, _d, _vp, _vp, _vp], _i32),

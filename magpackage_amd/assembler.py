@@ -253,6 +253,30 @@ def expand_material_map(map_or_array, tags: np.ndarray) -> np.ndarray:
     raise ValueError("Assembler:InvalidInput Invalid material array format.")
 
 
+def expand_source_map(map_or_array, tags: np.ndarray) -> np.ndarray:
+    """Assembler.expandSourceMap (src/Assembler.m:308-322): (Ne, 3) current
+    density, the memory of the reference's [3 x Ne] array.  Accepts a
+    {tag: (jx, jy, jz)} dict, a 3-vector, or a (3, Ne) / (Ne, 3) array."""
+    ne = len(tags)
+    out = np.zeros((ne, 3))
+    if isinstance(map_or_array, dict):
+        for k, v in map_or_array.items():
+            mask = tags == float(k)
+            v = np.asarray(v, dtype=np.float64).ravel()
+            if mask.any():
+                out[mask] = v[:3]
+        return out
+    a = np.asarray(map_or_array, dtype=np.float64)
+    if a.size == 3:
+        out[:] = a.ravel()
+        return out
+    if a.shape == (3, ne):
+        return np.ascontiguousarray(a.T)
+    if a.shape == (ne, 3):
+        return np.ascontiguousarray(a)
+    raise ValueError("Source must be [3x1] or [3xN].")
+
+
 class Assembler:
     def __init__(self, mesh: Mesh, dofHandler: DofHandler, config: FemConfig | None = None):
         self.Mesh = mesh
@@ -377,9 +401,83 @@ class Assembler:
         J = self._finish(plan, jv[: plan.nnz], False, drop_zeros) if calcJ else None
         return J, res[: plan.n]
 
+    def assembleNewtonSet(self, space, solutionA, matLibData, sigmaMap=None,
+                          precision: str = "fp64", outputs=("K", "M", "J", "R"), tables=None):
+        """One pass over the mesh for the Newton quantities at solution A:
+        secant stiffness K(nu(B_e)) (what assembleStiffness builds from the
+        per-element nu the B-H curve gives at A), the sigma-mass (sigma = 1
+        when sigmaMap is None), the Jacobian J and residual R of
+        assembleJacobian (src/Assembler.m:182-231).  Returns a dict of raw
+        structural-CSC value tensors (plan pattern, no zero dropping) for the
+        requested outputs; FP64 and FP32_LOCAL only."""
+        plan = self.plan(space)
+        dev = _dev()
+        if not torch.is_tensor(solutionA):
+            solutionA = torch.from_numpy(np.ascontiguousarray(solutionA, dtype=np.float64))
+        sol = solutionA.to(dev, dtype=torch.float64).contiguous()
+        if sol.numel() < plan.n:
+            sol = torch.cat([sol, torch.zeros(plan.n - sol.numel(), dtype=torch.float64, device=dev)])
+        mat, keep = tables if tables is not None else self.material_tables(matLibData)
+        ntag = int(mat.num_tags)
+        tmin, tmax = int(self.Mesh.RegionTags.min()), int(self.Mesh.RegionTags.max())
+        if tmin < 1 or tmax > ntag:
+            raise L.MpkError(L.MPK_ERR_INDEX, f"region tags [{tmin}, {tmax}] outside material table")
+        sig = None if sigmaMap is None else self._elem_vec(sigmaMap)
+        out = {}
+        for name in ("K", "M", "J"):
+            if name in outputs:
+                out[name] = torch.empty(max(plan.nnz, 1), dtype=torch.float64, device=dev)
+        if "R" in outputs:
+            out["R"] = torch.empty(max(plan.n, 1), dtype=torch.float64, device=dev)
+        qk = self.stiffness_quadrature()
+        qm = self.mass_quadrature(weighted=sigmaMap is not None)
+        L.check(L.lib.mpk_csc_newton_set(
+            plan.handle, L.ptr(self.Mesh.P_dev), L.ptr(self.Mesh.Tags_dev), L.ptr(sol),
+            C.byref(mat), L.ptr(sig), C.byref(qk), C.byref(qm), L.PRECISIONS[precision],
+            L.ptr(out.get("K")), L.ptr(out.get("M")), L.ptr(out.get("J")), L.ptr(out.get("R")),
+            L.stream_ptr()))
+        del keep
+        return {k: (v[: plan.n] if k == "R" else v[: plan.nnz]) for k, v in out.items()}
+
+    def source_quadrature(self) -> L.Quadrature:
+        return L.standard_quadrature(max(self.Config.DefaultQuadratureOrder, 2))
+
+    def assembleSource(self, space, sourceMap, out: torch.Tensor | None = None):
+        """F = assembleSource(space, sourceMap) (src/Assembler.m:78-97): the
+        dense [NumGlobalDofs] source vector on the device."""
+        plan = self.plan(space)
+        js = sourceMap if torch.is_tensor(sourceMap) else torch.from_numpy(
+            expand_source_map(sourceMap, self.Mesh.RegionTags)).to(_dev())
+        js = js.to(dtype=torch.float64).contiguous()
+        F = out if out is not None else torch.empty(max(plan.n, 1), dtype=torch.float64,
+                                                    device=_dev())
+        q = self.source_quadrature()
+        L.check(L.lib.mpk_csc_source(plan.handle, L.ptr(self.Mesh.P_dev), L.ptr(js), C.byref(q),
+                                     L.ptr(F), L.stream_ptr()))
+        return F[: plan.n]
+
+    def computeElementB(self, solutionA, space=None):
+        """PostProcessor.computeElementB (src/post/PostProcessor.m:64-126) for one
+        solution column: (Ne, 3) device tensor of B at the elements."""
+        key = space.toString() if space is not None else next(iter(self.DofHandler.DofMaps))
+        cell = self.DofHandler.DofMaps[key]
+        sol = solutionA if torch.is_tensor(solutionA) else torch.from_numpy(
+            np.ascontiguousarray(solutionA, dtype=np.float64))
+        sol = sol.to(_dev(), dtype=torch.float64).contiguous()
+        ne = self.Mesh.NumElements
+        B = torch.empty((max(ne, 1), 3), dtype=torch.float64, device=_dev())
+        q = self.stiffness_quadrature()
+        L.check(L.lib.mpk_element_b(L.ptr(self.Mesh.P_dev), self.Mesh.NumNodes,
+                                    L.ptr(self.Mesh.T_dev), L.ptr(cell.reshape(-1)), ne,
+                                    L.ptr(sol), sol.numel(), C.byref(q), L.ptr(B),
+                                    L.stream_ptr()))
+        return B[:ne]
+
     # snake_case aliases
     assemble_stiffness = assembleStiffness
     assemble_mass = assembleMass
     assemble_mass_weighted = assembleMassWeighted
     assemble_jacobian = assembleJacobian
     assemble_frequency_operator = assembleFrequencyOperator
+    assemble_source = assembleSource
+    compute_element_b = computeElementB

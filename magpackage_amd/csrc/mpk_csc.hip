@@ -292,4 +292,39 @@ int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
     return MPK_ERR_ARGS;
 }
 
+int mpk_csc_newton_set(const mpk_plan* plan, const double* P, const int32_t* tags,
+                       const double* sol_a, const mpk_material* mat, const double* sigma,
+                       const mpk_quadrature* quad_k, const mpk_quadrature* quad_m,
+                       int precision, double* k_vals, double* m_vals, double* j_vals,
+                       double* residual, void* stream) {
+    clear_error();
+    MPK_REQUIRE(plan && mat, MPK_ERR_ARGS, "NULL plan or material");
+    MPK_REQUIRE(plan->n == 0 || (tags && sol_a), MPK_ERR_ARGS, "NULL array argument");
+    MPK_REQUIRE(precision == MPK_PREC_FP64 || precision == MPK_PREC_FP32_LOCAL, MPK_ERR_ARGS,
+                "the fused Newton set has FP64 and FP32_LOCAL modes only (got %d)", precision);
+    CscQuad qk, qm;
+    if (int rc = make_csc_quad(quad_k, true, qk)) return rc;
+    if (int rc = make_csc_quad(quad_m, false, qm)) return rc;
+    MPK_REQUIRE(standard_curl_table(quad_k), MPK_ERR_ARGS,
+                "the fused Newton set needs the Nedelec P1 reference curls");
+    MaterialView mv{mat->linear_nu, mat->is_nonlinear, mat->max_bsq,    mat->breaks,
+                    mat->coefs,     mat->spline_start, mat->spline_count, mat->coef_start};
+    const int nbrk = (int)mat->num_breaks, ncoe = (int)mat->num_coefs;
+    const int ntags = (int)mat->num_tags;
+    MPK_REQUIRE(ntags >= 1, MPK_ERR_ARGS, "material table without tags");
+    const size_t extra = (nbrk + ncoe <= kMatLds) ? (size_t)(nbrk + ncoe) * 8 : 0;
+    hipStream_t st = as_stream(stream);
+    if (precision == MPK_PREC_FP64)
+        return launch_frame(plan, P,
+                            FrameNewton<MPK_PREC_FP64>{tags, sol_a, sigma, mv, ntags, nbrk, ncoe,
+                                                       k_vals, m_vals, j_vals, residual, qk, qm,
+                                                       0, nullptr, nullptr},
+                            sizeof(NewtonAcc), extra, st);
+    return launch_frame(plan, P,
+                        FrameNewton<MPK_PREC_FP32_LOCAL>{tags, sol_a, sigma, mv, ntags, nbrk,
+                                                         ncoe, k_vals, m_vals, j_vals, residual,
+                                                         qk, qm, 0, nullptr, nullptr},
+                        sizeof(NewtonAcc), extra, st);
+}
+
 }  // extern "C"

@@ -352,6 +352,124 @@ struct FrameJac {
     }
 };
 
+// Fused Newton set: in one pass over the plan, the secant stiffness K(nu(B))
+// (nu from the B-H table at the element's B, the K the reference assembles
+// with assembleStiffness from those per-element nu), the sigma-mass M, the
+// Jacobian J = K(nu) + 2 dnu (B.C_r)(B.C_c) and the residual R = K(nu) a.
+// Outputs may be NULL (skipped).  The curl-curl quadrature weight sum wk and
+// the mass moments come from the two tables the separate calls would use.
+struct NewtonAcc {
+    double k, m, j;
+};
+template <int kPrec>
+struct FrameNewton {
+    using Acc = NewtonAcc;
+    using Ids = FrameRows;
+    struct Elem {
+        JacElem jac;
+        double sigma;
+    };
+    struct Col {
+        double r;
+    };
+    const int32_t* __restrict__ tags;
+    const double* __restrict__ sol;
+    const double* __restrict__ sigma;  // NULL: sigma = 1
+    MaterialView mat;
+    int ntags, nbrk, ncoe;
+    double* __restrict__ out_k;
+    double* __restrict__ out_m;
+    double* __restrict__ out_j;
+    double* __restrict__ res;
+    CscQuad qk, qm;
+    int acc_bytes_max;
+    const double* brk_s;
+    const double* coe_s;
+    __device__ static Acc zero() { return NewtonAcc{0.0, 0.0, 0.0}; }
+    __device__ void block_init(unsigned char* lds) {
+        if (nbrk + ncoe <= kMatLds) {
+            double* tab = reinterpret_cast<double*>(lds);
+            for (int i = threadIdx.x; i < nbrk; i += kW) tab[i] = mat.brk[i];
+            for (int i = threadIdx.x; i < ncoe; i += kW) tab[nbrk + i] = mat.coe[i];
+            brk_s = tab;
+            coe_s = tab + nbrk;
+        } else {
+            brk_s = mat.brk;
+            coe_s = mat.coe;
+        }
+    }
+    __device__ Col col_begin() const { return Col{0.0}; }
+    __device__ void col_end(int64_t j, Col& cs) const {
+        if (res) res[j] = cs.r;
+    }
+    __device__ void store(int64_t p, Acc a) const {
+        if (out_k) out_k[p] = a.k;
+        if (out_m) out_m[p] = a.m;
+        if (out_j) out_j[p] = a.j;
+    }
+    __device__ Ids ids(const TileCtx& tc, int cb, const FPos& pos) const {
+        Ids d;
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const int p = fpos(pos, rp);
+            d.row[rp] = (p != 0xff) ? tc.rowind[cb + p] : -1;
+        }
+        return d;
+    }
+    __device__ Elem elem(uint32_t e, const Ids& d) const {
+        Elem el;
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) el.jac.a[rp] = d.row[rp] >= 0 ? sol[d.row[rp]] : 0.0;
+        const int tag = tags[e];
+        el.jac.tag = tag < 1 ? 1 : (tag > ntags ? ntags : tag);
+        el.sigma = sigma ? sigma[e] : 1.0;
+        return el;
+    }
+    __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
+                            Acc* acc, Col& cs) const {
+        const bool has_m = fabs(el.sigma) > 1e-12;
+        const FrameGeom g = frame_geom(nodes, has_m);
+        const double two_inv = 2.0 / g.det;
+        Vec3 Bv{0.0, 0.0, 0.0};
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const double ap = el.jac.a[rp] * fsign(f, rp);
+            Bv.x += ap * g.u[rp].x;
+            Bv.y += ap * g.u[rp].y;
+            Bv.z += ap * g.u[rp].z;
+        }
+        Bv.x *= two_inv;
+        Bv.y *= two_inv;
+        Bv.z *= two_inv;
+        double nu, dnu;
+        eval_nu(dot(Bv, Bv), el.jac.tag, mat, brk_s, coe_s, nu, dnu);
+        const double inv = 1.0 / fabs(g.det);
+        const double wd = qk.wsum * fabs(g.det);
+        const double f0 = fsign(f, 0);
+        const double BC0 = two_inv * dot(Bv, g.u[0]);
+        cs.r += nu * BC0 * wd * f0;
+        const double ks = 4.0 * qk.wsum * nu * inv * f0;  // = nu * (4/det^2) * wd
+        const double ms = el.sigma * inv * f0;
+        double m[6] = {0, 0, 0, 0, 0, 0};
+        if (has_m && out_m) mass_col<kPrec>(g, qm, m);
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const int p = fpos(pos, rp);
+            const double s = fsign(f, rp);
+            const double kv = curl_dot<kPrec>(g, rp) * (ks * s);
+            const double BCr = two_inv * dot(Bv, g.u[rp]);
+            const double jv = kv + 2.0 * dnu * BCr * BC0 * wd * (s * f0);
+            if (p != 0xff) {
+                Acc a = acc[p];
+                a.k += kv;
+                a.m += m[rp] * (ms * s);
+                a.j += jv;
+                acc[p] = a;
+            }
+        }
+    }
+};
+
 // -------------------------------------------------------------- drivers --
 // Frame driver, element order.  Per lane, a 4-stage pipeline:
 //   stage 3: frame record                            for k + 3

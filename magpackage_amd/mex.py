@@ -119,6 +119,24 @@ def assemble_jacobian_kernel_mex(*args):
             Ri[: 6 * ne].cpu().numpy(), Rv[: 6 * ne].cpu().numpy())
 
 
+def assemble_source_kernel_mex(*args):
+    """[I, V] = assemble_source_kernel_mex(P, T, Dofs, Signs, Js, Qw, ValRef)
+    (src/mex/assemble_source_kernel_mex.cpp:16-120); Js is [3 x Ne]."""
+    _arity(args, 7)
+    P, T, Dofs, Signs, Js, Qw, ValRef = args
+    ne = _ncols(T)
+    nn = _colmajor(P).size // 3
+    js = np.asarray(Js, dtype=np.float64)
+    if js.ndim != 2 or js.shape[0] != 3 or js.shape[1] != ne:
+        raise L.MpkError(L.MPK_ERR_ARGS, "Js must be [3 x NumElems]")
+    q = _quad(Qw, ValRef, False)
+    d = [_dev(_colmajor(x)) for x in (P, T, Dofs, Signs, js)]
+    Io, Vo = _out(6 * ne), _out(6 * ne)
+    L.check(L.lib.mpk_source_coo(L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]),
+                                 L.ptr(d[4]), C.byref(q), L.ptr(Io), L.ptr(Vo), L.stream_ptr()))
+    return tuple(t[: 6 * ne].cpu().numpy() for t in (Io, Vo))
+
+
 def sparse_gpu(I, J, V, m, n, drop_zeros: bool = True):
     """Returns (colptr int64 [n+1], rowind int32 0-based [nnz], vals [nnz]) on the host."""
     I, J, V = _colmajor(I), _colmajor(J), _colmajor(V)

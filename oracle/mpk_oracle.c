@@ -21,6 +21,8 @@
  *   orc_mass_kernel      src/mex/assemble_mass_kernel_mex.cpp:16-127
  *   orc_eval_nu          src/mex/MexMaterialUtils.hpp:24-87 (evaluate_nu_derivative)
  *   orc_jacobian_kernel  src/mex/assemble_jacobian_kernel_mex.cpp:7-153
+ *   orc_source_kernel    src/mex/assemble_source_kernel_mex.cpp:16-120
+ *   orc_element_b        src/post/PostProcessor.m:64-126 (computeElementB)
  *   orc_sparse           src/Assembler.m:37-38  (valid = I>0 & J>0; MATLAB sparse():
  *                        duplicates summed, CSC with sorted rows, zeros dropped;
  *                        summation order = triplet order, an assumption: MATLAB's
@@ -511,4 +513,75 @@ void orc_boundary_edges(const int32_t* T, int64_t ne, const int32_t* edges, int6
         i = j;
     }
     free(f);
+}
+
+/* ------------------------------------------------------------------ */
+/* Source vector F = (J_s, N_i): assemble_source_kernel_mex            */
+/* ------------------------------------------------------------------ */
+/*
+ * Js: [3 x ne] current density per element (physical space).  Output: 6 ne
+ * pairs (I, V), slot 6e + i = (dof_i, F_i * s_i) (:104-114).  An element
+ * whose |Js| components are all <= 1e-12 contributes zeros (:80).  The
+ * integrand uses J_inv * Js dotted with the reference values (:68-76).
+ */
+void orc_source_kernel(const double* P, const double* T, const double* Dofs,
+                       const double* Signs, const double* Js, const double* Qw, int64_t nq,
+                       const double* RefVal, int64_t ne, double* I, double* V) {
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < ne; e++) {
+        double p[12], J[9], Ji[9];
+        orc_gather(P, T, e, p);
+        double detJ = orc_jacobian_3d(p, J);
+        double absDetJ = fabs(detJ);
+        orc_inverse_3x3(J, detJ, Ji);
+        double jp[3] = {Js[0 + e * 3], Js[1 + e * 3], Js[2 + e * 3]};
+        double jt[3];
+        for (int i = 0; i < 3; i++) jt[i] = Ji[3 * i + 0] * jp[0] + Ji[3 * i + 1] * jp[1] + Ji[3 * i + 2] * jp[2];
+        double F[6] = {0, 0, 0, 0, 0, 0};
+        if (fabs(jp[0]) > 1e-12 || fabs(jp[1]) > 1e-12 || fabs(jp[2]) > 1e-12) {
+            for (int64_t q = 0; q < nq; q++) {
+                double w = Qw[q] * absDetJ;
+                long qo = q * 18;
+                for (int i = 0; i < 6; i++) {
+                    double rn[3] = {RefVal[i + 0 + qo], RefVal[i + 6 + qo], RefVal[i + 12 + qo]};
+                    double val = jt[0] * rn[0] + jt[1] * rn[1] + jt[2] * rn[2];
+                    F[i] += val * w;
+                }
+            }
+        }
+        for (int i = 0; i < 6; i++) {
+            I[6 * e + i] = (double)(long)Dofs[i + e * 6];
+            V[6 * e + i] = F[i] * Signs[i + e * 6];
+        }
+    }
+}
+
+/*
+ * Element flux density at the centroid, B_e = C_phy' * (A(dofs) .* s) with
+ * C_phy = (C_ref * J') / det (PostProcessor.m:103-123); C_ref is the constant
+ * Nedelec P1 curl table.  Inactive DOFs (0) contribute 0.  B: [3 x ne].
+ */
+void orc_element_b(const double* P, const double* T, const double* Dofs, const double* Signs,
+                   const double* sol, const double* CurlRef, int64_t ne, double* B) {
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < ne; e++) {
+        double p[12], J[9];
+        orc_gather(P, T, e, p);
+        double detJ = orc_jacobian_3d(p, J);
+        double invDet = 1.0 / detJ;
+        double b[3] = {0, 0, 0};
+        for (int i = 0; i < 6; i++) {
+            long d = (long)Dofs[i + e * 6];
+            double a = d > 0 ? sol[d - 1] * Signs[i + e * 6] : 0.0;
+            /* row i of C_ref * J': sum_k C_ref(i,k) J(c,k), then / det */
+            for (int c = 0; c < 3; c++) {
+                double cp = (CurlRef[i + 0] * J[3 * c + 0] + CurlRef[i + 6] * J[3 * c + 1] +
+                             CurlRef[i + 12] * J[3 * c + 2]) * invDet;
+                b[c] += cp * a;
+            }
+        }
+        B[3 * e + 0] = b[0];
+        B[3 * e + 1] = b[1];
+        B[3 * e + 2] = b[2];
+    }
 }

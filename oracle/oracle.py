@@ -125,6 +125,26 @@ def jacobian(P, T, Dofs, Signs, Tags, SolA, Qw, CurlRef, packed, calcJ=True):
     return I[:n36], J[:n36], V[:n36], Ri, Rv
 
 
+def source(P, T, Dofs, Signs, Js, Qw, ValRef):
+    """[I, V] = assemble_source_kernel_mex(P, T, Dofs, Signs, Js, Qw, ValRef); Js (Ne, 3)."""
+    Pp = _d(P); Tp = _d(T); Dp = _d(Dofs); Sp = _d(Signs); Jp = _d(Js)
+    Qp = _d(Qw); Rp = _d(ValRef)
+    ne = Tp[0].size // 4
+    I = np.zeros(6 * ne); V = np.zeros(6 * ne)
+    _lib.orc_source_kernel(Pp[1], Tp[1], Dp[1], Sp[1], Jp[1], Qp[1], _i64(Qp[0].size), Rp[1],
+                           _i64(ne), I.ctypes.data_as(_dp), V.ctypes.data_as(_dp))
+    return I, V
+
+
+def element_b(P, T, Dofs, Signs, sol, CurlRef):
+    """PostProcessor.computeElementB for one solution column: B (Ne, 3)."""
+    Pp = _d(P); Tp = _d(T); Dp = _d(Dofs); Sp = _d(Signs); sp = _d(sol); Cp = _d(CurlRef)
+    ne = Tp[0].size // 4
+    B = np.zeros((ne, 3))
+    _lib.orc_element_b(Pp[1], Tp[1], Dp[1], Sp[1], sp[1], Cp[1], _i64(ne), B.ctypes.data_as(_dp))
+    return B
+
+
 def eval_nu(s, tags, packed):
     s, sp = _d(s)
     tags, tp = _i(tags)

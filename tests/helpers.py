@@ -90,3 +90,29 @@ def csc_equal_bits(A, B):
     A.sort_indices(); B.sort_indices()
     return (np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
             and np.array_equal(A.data.view(np.uint64), B.data.view(np.uint64)))
+
+
+def element_B(P, T, Dofs, Signs, sol):
+    """Per-element B = sum_i a_i s_i curl W_i (vectorised Whitney curls,
+    2 grad l_a x grad l_b); P (Nn, 3), T (Ne, 4), Dofs / Signs (Ne, 6), ids
+    1-based (the memory of the MATLAB [3 x Nn], [4 x Ne], [6 x Ne] arrays)."""
+    X = P[(T - 1).astype(np.int64)]  # (Ne, 4, 3)
+    Jm = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], axis=2)
+    Jit = np.transpose(np.linalg.inv(Jm), (0, 2, 1))
+    g_ref = np.array([[-1.0, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    g = np.einsum("ij,ekj->eik", g_ref, Jit)  # (Ne, 4, 3) physical gradients
+    d = Dofs.astype(np.int64)
+    a = np.where(d > 0, sol[np.maximum(d, 1) - 1], 0.0) * Signs
+    B = np.zeros((T.shape[0], 3))
+    for i, (p, q) in enumerate(EDGE_NODES):
+        B += a[:, i:i + 1] * 2.0 * np.cross(g[:, p], g[:, q])
+    return B
+
+
+def csc_from_plan(plan, vals):
+    """scipy CSC matrix of structural-pattern values from the plan."""
+    import scipy.sparse as sp
+    colptr, rowind = plan.pattern()
+    v = vals.detach().cpu().numpy() if hasattr(vals, "detach") else np.asarray(vals)
+    return sp.csc_matrix((v, rowind.cpu().numpy(), colptr.cpu().numpy()),
+                         shape=(plan.n, plan.n))

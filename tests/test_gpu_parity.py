@@ -12,8 +12,8 @@ import pytest
 from oracle import oracle as O
 from magpackage_amd.material import create_linear, marrocco, pack_material_data, MU0
 from magpackage_amd.meshgen import kuhn_box, kuhn_counts
-from tests.helpers import (config1_mesh, config1_nu, csc_equal_bits, gradient_matrix,
-                           mass_tables, random_tets, rel_err, stiffness_tables)
+from tests.helpers import (config1_mesh, config1_nu, csc_equal_bits, csc_from_plan, element_B,
+                           gradient_matrix, mass_tables, random_tets, rel_err, stiffness_tables)
 
 pytestmark = pytest.mark.gpu
 
@@ -218,6 +218,46 @@ def test_csc_mass(cfg1, prec, driver):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
+def test_source_vector_bit_exact(cfg1):
+    """assemble_source_kernel_mex (COO) and assembleSource (per-DOF sums in
+    triplet order) against the oracle + sparse(I, 1, V, N, 1), bitwise."""
+    from magpackage_amd.mex import assemble_source_kernel_mex
+    mesh, M, dh, space, asm, mi = cfg1
+    qm, val, _ = mass_tables(2)
+    rng = np.random.default_rng(31)
+    Js = rng.normal(size=(mesh.num_elems, 3)) * 3e6
+    Js[mesh.RegionTags == 2] = 0.0
+    I, V = O.source(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], Js, qm, val)
+    n = mi["ndof"]
+    # MATLAB-shaped arguments: P [3 x Nn], T [4 x Ne], Dofs / Signs [6 x Ne], Js [3 x Ne]
+    Ig, Vg = assemble_source_kernel_mex(mi["P"].T, mi["T"].T, mi["Dofs"].T, mi["Signs"].T,
+                                        Js.T, qm, val)
+    assert np.array_equal(Ig, I) and np.array_equal(Vg.view(np.uint64), V.view(np.uint64))
+    c, r, v = O.sparse(I, np.ones_like(I), V, n, 1, drop_zeros=False)
+    Fref = np.zeros(n)
+    Fref[r] = v
+    F = asm.assembleSource(space, Js).cpu().numpy()
+    assert np.array_equal(F.view(np.uint64), Fref.view(np.uint64))
+    # a region map {tag: (jx, jy, jz)} as Assembler.expandSourceMap takes it
+    F2 = asm.assembleSource(space, {1: (0.0, 0.0, 3e6)}).cpu().numpy()
+    Jm = np.zeros((mesh.num_elems, 3))
+    Jm[mesh.RegionTags == 1, 2] = 3e6
+    I2, V2 = O.source(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], Jm, qm, val)
+    F2ref = np.zeros(n)
+    c, r, v = O.sparse(I2, np.ones_like(I2), V2, n, 1, drop_zeros=False)
+    F2ref[r] = v
+    assert np.array_equal(F2.view(np.uint64), F2ref.view(np.uint64))
+
+
+def test_element_b_bit_exact(cfg1):
+    mesh, M, dh, space, asm, mi = cfg1
+    _, curl, _ = stiffness_tables()
+    a = np.random.default_rng(32).normal(size=mi["ndof"])
+    Bref = O.element_b(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a, curl)
+    B = asm.computeElementB(a, space).cpu().numpy()
+    assert np.array_equal(B.view(np.uint64), Bref.view(np.uint64))
+
+
 def test_csc_preloaded_coordinates(cfg1, prec, driver):
     """P == NULL reads the coordinates loaded by mpk_plan_set_coords: values
     are bit-identical to the call given P; FP64_EXACT and a plan that never
@@ -306,6 +346,50 @@ def test_csc_jacobian(cfg1, prec, driver):
         assert rel_err(Rg, Rref) < TOL[prec]
     _, R0 = asm.assembleJacobian(space, a, lib, calcJ=False, precision=prec)
     assert np.array_equal(R0.cpu().numpy(), Rg)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
+def test_csc_newton_set(cfg1, prec, driver):
+    """Fused K(nu(B)) + M_sigma + J + R against the oracle's separate kernels:
+    J and R from the Jacobian kernel, M from the mass kernel, K from the
+    curl-curl kernel with nu_e = eval_nu(|B_e|^2) at the element's B."""
+    mesh, M, dh, space, asm, mi = cfg1
+    lib = {1: create_linear(1.0), 2: marrocco()}
+    packed = pack_material_data(lib)
+    qw, curl, _ = stiffness_tables()
+    qm, val, _ = mass_tables(2)
+    n = mi["ndof"]
+    a = np.random.default_rng(18).normal(size=n) * 0.004
+    tags = mesh.RegionTags.astype(np.float64)
+    sig = np.where(mesh.RegionTags == 2, 2e6, 0.0)
+    out = asm.assembleNewtonSet(space, a, lib, sigmaMap=sig, precision=prec)
+    plan = asm.plan(space)
+    I, J, V, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, a, qw, curl,
+                                 packed)
+    Jref = O.sparse_scipy(I, J, V, n, n)
+    Rref = np.zeros(n)
+    np.add.at(Rref, Ri.astype(np.int64) - 1, Rv)
+    B = element_B(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a)
+    nu_e, _ = O.eval_nu((B * B).sum(axis=1), mesh.RegionTags, packed)
+    Kref = O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu_e, qw,
+                                       curl), n, n)
+    Mref = O.sparse_scipy(*O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sig, qm, val),
+                          n, n)
+    tol = TOL[prec]
+    for name, ref in (("K", Kref), ("M", Mref), ("J", Jref)):
+        A = csc_from_plan(plan, out[name])
+        assert abs(A - ref).max() / abs(ref).max() < tol, name
+    assert rel_err(out["R"].cpu().numpy(), Rref) < tol
+    # subsets of outputs give the same values
+    only = asm.assembleNewtonSet(space, a, lib, sigmaMap=sig, precision=prec, outputs=("J",))
+    assert torch_equal_bits(only["J"], out["J"])
+    with pytest.raises(Exception, match="FP64 and FP32_LOCAL"):
+        asm.assembleNewtonSet(space, a, lib, precision="fp64_exact")
+
+
+def torch_equal_bits(a, b):
+    import torch
+    return torch.equal(a.view(torch.int64), b.view(torch.int64))
 
 
 # --------------------------------------- size-independent properties -------

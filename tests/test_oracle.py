@@ -342,3 +342,48 @@ def test_distribute_dofs_restricted():
     assert np.array_equal(emap[used - 1], 7 + np.arange(1, nd + 1))
     assert np.count_nonzero(emap) == nd
     assert np.array_equal(cell, emap[t2e - 1])
+
+
+# ---------------------------------------------------- source and B ---------
+def test_source_closed_form_random_tets():
+    """F_i = s_i Js . int W_i = s_i Js . (|T|/4)(grad l_b - grad l_a): the
+    4-point rule integrates the linear Whitney functions exactly (its
+    truncated constants still sum a + 3b = 1)."""
+    P, T = random_tets(150, seed=21)
+    ne = T.shape[0]
+    rng = np.random.default_rng(22)
+    S = rng.choice([-1.0, 1.0], size=(ne, 6))
+    D = np.arange(1, 6 * ne + 1, dtype=np.float64).reshape(ne, 6)
+    Js = rng.normal(size=(ne, 3)) * 3e6
+    Js[::7] = 0.0  # zero-source elements give zeros
+    qw, val, _ = mass_tables(2)
+    I, V = O.source(P, T.astype(np.float64), D, S, Js, qw, val)
+    assert np.array_equal(I, D.ravel())
+    worst = 0.0
+    for e in range(ne):
+        g, det = grads(P[T[e] - 1])
+        vol = abs(det) / 6.0
+        ref = np.array([S[e, i] * Js[e] @ (g[b] - g[a]) * vol / 4.0
+                        for i, (a, b) in enumerate(EDGE_NODES)])
+        if not Js[e].any():
+            assert np.all(V[6 * e:6 * e + 6] == 0.0)
+            continue
+        worst = max(worst, rel_err(V[6 * e:6 * e + 6], ref))
+    assert worst < 1e-12
+
+
+def test_element_b_against_whitney_curls():
+    """computeElementB restatement vs curl W_ab = 2 grad l_a x grad l_b."""
+    P, T = random_tets(80, seed=23)
+    ne = T.shape[0]
+    rng = np.random.default_rng(24)
+    S = rng.choice([-1.0, 1.0], size=(ne, 6))
+    D = np.arange(1, 6 * ne + 1, dtype=np.float64).reshape(ne, 6)
+    D[5, 2] = 0.0  # an inactive DOF contributes nothing
+    sol = rng.normal(size=6 * ne)
+    _, curl, _ = stiffness_tables()
+    B = O.element_b(P, T.astype(np.float64), D, S, sol, curl)
+    for e in range(ne):
+        c, _ = whitney_curls(P[T[e] - 1])
+        a = np.where(D[e] > 0, sol[np.maximum(D[e].astype(int), 1) - 1], 0.0) * S[e]
+        assert rel_err(B[e], a @ c) < 1e-12
