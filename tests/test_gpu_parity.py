@@ -217,7 +217,6 @@ def test_csc_mass(cfg1, prec, driver):
             assert abs(Mg - ref).max() / abs(ref).max() < TOL[prec]
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
 def test_source_vector_bit_exact(cfg1):
     """assemble_source_kernel_mex (COO) and assembleSource (per-DOF sums in
     triplet order) against the oracle + sparse(I, 1, V, N, 1), bitwise."""
@@ -258,6 +257,7 @@ def test_element_b_bit_exact(cfg1):
     assert np.array_equal(B.view(np.uint64), Bref.view(np.uint64))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
 def test_csc_preloaded_coordinates(cfg1, prec, driver):
     """P == NULL reads the coordinates loaded by mpk_plan_set_coords: values
     are bit-identical to the call given P; FP64_EXACT and a plan that never
