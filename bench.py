@@ -55,6 +55,195 @@ def freq_workload(n=128, seed=2):
     return km, nu, sigma, 2 * np.pi * 50.0
 
 
+def algorithmic_bytes_newton(nn, ne, n, nnz):
+    """Fused Newton set: coordinates 24*Nn, connectivity 16*Ne, edge DOFs
+    24*Ne, sigma 8*Ne, tags 4*Ne, A 8*N in; K, M, J 8*nnz each and R 8*N out."""
+    return 24 * nn + 16 * ne + 24 * ne + 8 * ne + 4 * ne + 8 * n + 3 * 8 * nnz + 8 * n
+
+
+def newton_workload(nx=100, ny=100, nz=67, seed=3):
+    """100x100x67-cube jittered Kuhn box (cube edge 0.01), an iron core
+    (centroid in [0.3,0.7]x[0.3,0.7]x[0.2,0.47], Marrocco curve, sigma 2e6)
+    in air (nu0 = 1/mu0)."""
+    from magpackage_amd.material import create_linear, marrocco
+    from magpackage_amd.meshgen import kuhn_box, tag_box
+    km = kuhn_box(nx, ny, nz, lengths=(nx / 100.0, ny / 100.0, nz / 100.0), seed=seed)
+    tag_box(km, (0.3, 0.3, 0.2), (0.7, 0.7, 0.47), 2)
+    lib = {1: create_linear(1.0), 2: marrocco()}
+    sigma = np.where(km.RegionTags == 2, 2e6, 0.0)
+    return km, lib, sigma
+
+
+def cpu_baseline_newton(sample: int, reps: int):
+    """Reference path on the host for the Newton set: the oracle's MEX-order
+    Jacobian (J, R), curl-curl (K at the element nu) and mass kernels on
+    OpenMP threads + sparse() of each, on a bounded sample of the workload."""
+    from oracle import oracle as O
+    from magpackage_amd.fem_tables import get_quadrature_data, matlab_flat, nedelec_tet_p1
+    from magpackage_amd.material import pack_material_data
+    nz = max(2, int(round(sample * 0.67)))
+    km, lib, sigma = newton_workload(sample, sample, nz, seed=3)
+    mi = O.mesh_inputs(km)
+    n = mi["ndof"]
+    packed = pack_material_data(lib)
+    q_pts, qw = get_quadrature_data("tet", 2)
+    val, curl = nedelec_tet_p1(q_pts)
+    val, curl = matlab_flat(val), matlab_flat(curl)
+    a = np.random.default_rng(3).normal(size=n) * 1e-3
+    tags = km.RegionTags.astype(np.float64)
+
+    def run():
+        t0 = time.perf_counter()
+        I, J, V, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, a, qw,
+                                     curl, packed)
+        Jm = O.sparse_scipy(I, J, V, n, n)
+        R = np.zeros(n)
+        np.add.at(R, Ri.astype(np.int64) - 1, Rv)
+        B = O.element_b(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a, curl)
+        nu, _ = O.eval_nu((B * B).sum(axis=1), km.RegionTags, packed)
+        K = O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl),
+                           n, n)
+        M = O.sparse_scipy(*O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sigma, qw, val),
+                           n, n)
+        t1 = time.perf_counter()
+        return t1 - t0, (Jm.nnz, K.nnz, M.nnz, R.size)
+
+    run()
+    times = [run()[0] for _ in range(reps)]
+    t = float(np.median(times))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": km.num_elems / t,
+        "unit": "tets/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"Kuhn {sample}x{sample}x{nz} box ({km.num_elems} tets, same core and "
+                   f"curve), Jacobian + B + curl-curl + mass kernels on {cores} OpenMP threads "
+                   f"+ serial sparse() of K, M, J, median of {reps} after 1 warm-up: {t:.3f} s"),
+    }
+
+
+def run_newton(args):
+    """K(nu(B)) + M_sigma + J + R in one fused pass, FP32-local by default."""
+    import torch
+    from magpackage_amd import _lib as L
+    from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    km, lib, sigma = newton_workload()
+    log(f"[bench] mesh 100x100x67: {km.num_elems} tets, {km.num_nodes} nodes "
+        f"({time.perf_counter() - t0:.1f} s)")
+    mesh = Mesh.from_kuhn(km)
+    dh = DofHandler(mesh)
+    space = FunctionSpace("Nedelec", 1)
+    dh.distributeDofs(space)
+    asm = Assembler(mesh, dh)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    plan = asm.plan(space)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    n = plan.n
+    dev = torch.device("cuda", 0)
+    # A ~ N(0, 1) (seed 3), scaled so that max |B_e| = 2.2 T
+    a = torch.from_numpy(np.random.default_rng(3).normal(size=n)).to(dev)
+    bmax = float(asm.computeElementB(a, space).norm(dim=1).max())
+    a *= 2.2 / bmax
+    bn = asm.computeElementB(a, space).norm(dim=1)
+    log(f"[bench] plan {1e3 * (t3 - t2):.1f} ms: {n} dofs, nnz {plan.nnz}; "
+        f"|B| in [{float(bn.min()):.3g}, {float(bn.max()):.3g}] T")
+    mat, keep = asm.material_tables(lib)
+    sg = torch.from_numpy(sigma).to(dev)
+    kv = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
+    mv = torch.empty_like(kv)
+    jv = torch.empty_like(kv)
+    rv = torch.empty(n, dtype=torch.float64, device=dev)
+    qk = asm.stiffness_quadrature()
+    qm = asm.mass_quadrature(weighted=True)
+    prec = L.PRECISIONS[args.precision]
+    stream = torch.cuda.current_stream()
+    sp = L.stream_ptr(stream)
+    pd = L.ptr(mesh.P_dev)
+
+    def step(ev_a=None, ev_b=None):
+        L.check(L.lib.mpk_plan_set_coords(plan.handle, pd, sp))
+        if ev_a is not None:
+            ev_a.record(stream)
+        L.check(L.lib.mpk_csc_newton_set(plan.handle, None, L.ptr(mesh.Tags_dev), L.ptr(a),
+                                         C.byref(mat), L.ptr(sg), C.byref(qk), C.byref(qm), prec,
+                                         L.ptr(kv), None if args.split_m else L.ptr(mv),
+                                         L.ptr(jv), L.ptr(rv), sp))
+        if args.split_m:
+            L.check(L.lib.mpk_csc_mass(plan.handle, None, L.ptr(sg), C.byref(qm), prec,
+                                       L.ptr(mv), sp))
+        if ev_b is not None:
+            ev_b.record(stream)
+
+    ms_per, ms_kernel = timed_steps(step, stream, args.steps, args.warmup)
+    ne, nn, nnz = km.num_elems, km.num_nodes, plan.nnz
+    value = ne / (ms_per * 1e-3)
+    alg = algorithmic_bytes_newton(nn, ne, n, nnz)
+    achieved = alg / (ms_kernel * 1e-3) / 1e9
+    log(f"[bench] {args.steps} steps: {ms_per:.4f} ms/step (events), "
+        f"{'K+J+R pass + M pass' if args.split_m else 'fused K+M+J+R'} "
+        f"{ms_kernel:.4f} ms, {value:.4g} tets/s, {achieved:.0f} GB/s algorithmic")
+    if args.calib:
+        calib_copy(dev)
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline on a {args.cpu_sample}-cube sample ...")
+        cpu = cpu_baseline_newton(args.cpu_sample, args.cpu_reps)
+        log(f"[bench] cpu: {cpu['value']:.4g} tets/s")
+    workload = "kuhn100x100x67_newton_kmjr"
+    print(json.dumps({
+        "metric": METRIC, "value": value, "unit": "tets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-local/f64-acc",
+        "data": "synthetic (seeded jittered Kuhn box, Gaussian A scaled to max |B| = 2.2 T)",
+        "config": {"workload": workload,
+                   "operator": "K(nu(B)) + M_sigma + J + R, fused, FP64 CSC values",
+                   "tets": ne, "nodes": nn, "edge_dofs": n, "nnz": int(nnz),
+                   "precision": args.precision, "plan_build_ms": round(1e3 * (t3 - t2), 1)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(workload),
+                     "algorithmic_bytes": alg, "kernel_ms": ms_kernel},
+        "cpu_baseline": cpu,
+    }), flush=True)
+
+
+def timed_steps(step, stream, steps, warmup):
+    """Runs warmup + timed steps; returns (ms per step over the timed region,
+    mean ms of the bracketed kernel launches)."""
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    eva = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    evb = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ev0.record(stream)
+    for i in range(steps):
+        step(eva[i], evb[i])
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_per = ev0.elapsed_time(ev1) / steps
+    ms_kernel = float(np.mean([x.elapsed_time(y) for x, y in zip(eva, evb)]))
+    return ms_per, ms_kernel
+
+
+def calib_copy(dev):
+    import torch
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    log("[bench] calibration copy: 3 x 512 MiB read + 512 MiB write")
+
+
 def cpu_baseline(sample_n: int, reps: int):
     """Reference path on the host: MEX-order COO kernels (oracle, OpenMP) +
     sparse() for K and M_sigma + K + 1j*omega*M on a bounded sample."""
@@ -109,8 +298,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="freq128", choices=["freq128", "newton4m"],
+                    help="freq128: K + 1j*w*M_sigma complex128 on the 128^3 box (default); "
+                         "newton4m: fused K(nu(B)) + M_sigma + J + R on the 100x100x67 box")
     ap.add_argument("--n", type=int, default=128, help="cubes per axis of the Kuhn box")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp64_exact", "fp32_local"])
+    ap.add_argument("--split-m", action="store_true",
+                    help="newton4m: K, J, R in one pass and M_sigma in a second one")
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp64_exact", "fp32_local"],
+                    help="default fp64 (freq128) / fp32_local (newton4m)")
     ap.add_argument("--cpu-sample", type=int, default=80)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -124,6 +319,14 @@ def main():
         print(json.dumps({"error": "this engine is measured on one GPU only; "
                                    "run with --gpus 1 (multi-GPU sharding is a design note)"}))
         sys.exit(2)
+    if args.workload == "newton4m":
+        args.precision = args.precision or "fp32_local"
+        if args.precision == "fp64_exact":
+            ap.error("the fused Newton set has fp64 and fp32_local modes")
+        if args.cpu_sample == 80:
+            args.cpu_sample = 64
+        return run_newton(args)
+    args.precision = args.precision or "fp64"
 
     import torch
     from magpackage_amd import _lib as L
@@ -173,39 +376,17 @@ def main():
         if ev_b is not None:
             ev_b.record(stream)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    eva = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    evb = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    tw0 = time.perf_counter()
-    ev0.record(stream)
-    for i in range(args.steps):
-        step(eva[i], evb[i])
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    tw1 = time.perf_counter()
-    ms_total = ev0.elapsed_time(ev1)
-    ms_per = ms_total / args.steps
-    ms_kernel = float(np.mean([a.elapsed_time(b) for a, b in zip(eva, evb)]))
+    ms_per, ms_kernel = timed_steps(step, stream, args.steps, args.warmup)
     ne, nn, nnz = km.num_elems, km.num_nodes, plan.nnz
     value = ne / (ms_per * 1e-3)
     alg = algorithmic_bytes_freq(nn, ne, nnz)
     achieved = alg / (ms_kernel * 1e-3) / 1e9
     log(f"[bench] {args.steps} steps: {ms_per:.4f} ms/step (events), assembly kernel "
-        f"{ms_kernel:.4f} ms, wall {1e3 * (tw1 - tw0):.1f} ms, {value:.4g} tets/s, "
+        f"{ms_kernel:.4f} ms, {value:.4g} tets/s, "
         f"{achieved:.0f} GB/s algorithmic")
 
     if args.calib:
-        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 1.0, out=b)
-        torch.cuda.synchronize()
-        log("[bench] calibration copy: 3 x 512 MiB read + 512 MiB write")
-        del a, b
+        calib_copy(dev)
 
     cpu = None
     if not args.no_cpu:

@@ -92,6 +92,88 @@ int lds_bytes(const mpk_plan* plan, Op& op, size_t acc_size, size_t extra, size_
     return MPK_OK;
 }
 
+int load_coords(const mpk_plan* plan, const double* P, hipStream_t st);
+
+// Element state for the Newton kernels, one lane per element: B = J v / det
+// with v = sum_i a_i s_i c_i over the constant reference curls c_i
+// (nedelec_tet_p1.m:48; the Piola map of assemble_jacobian_kernel_mex.cpp:
+// 94-113), then (nu, dnu) = evaluate_nu_derivative(|B|^2, tag)
+// (MexMaterialUtils.hpp:24-87) with the B-H tables staged in LDS.
+constexpr int kStateBlock = 256;
+
+__global__ void __launch_bounds__(kStateBlock)
+k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
+             const double4* __restrict__ P4, int64_t ne, const double* __restrict__ sol,
+             const int32_t* __restrict__ tags, const double* __restrict__ sigma, MaterialView mat,
+             int ntags, int nbrk, int ncoe, double4* __restrict__ s0, double2* __restrict__ s1) {
+    __shared__ double tab[kMatLds];
+    const double* brk = mat.brk;
+    const double* coe = mat.coe;
+    if (nbrk + ncoe <= kMatLds) {
+        for (int i = threadIdx.x; i < nbrk; i += kStateBlock) tab[i] = mat.brk[i];
+        for (int i = threadIdx.x; i < ncoe; i += kStateBlock) tab[nbrk + i] = mat.coe[i];
+        brk = tab;
+        coe = tab + nbrk;
+    }
+    __syncthreads();
+    const int64_t e = (int64_t)blockIdx.x * kStateBlock + threadIdx.x;
+    if (e >= ne) return;
+    const int4 t = T[e];
+    const int nd[4] = {t.x, t.y, t.z, t.w};
+    const Vec3 x0 = ld_node(P4, t.x), x1 = ld_node(P4, t.y), x2 = ld_node(P4, t.z),
+               x3 = ld_node(P4, t.w);
+    const Vec3 d1 = sub(x1, x0), d2 = sub(x2, x0), d3 = sub(x3, x0);
+    const double det = dot(d1, cross(d2, d3));
+    double al[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const int d = cell[6 * e + i];
+        const double s = nd[edge_a(i)] <= nd[edge_b(i)] ? 1.0 : -1.0;
+        al[i] = d >= 0 ? sol[d] * s : 0.0;
+    }
+    // v = sum_i al_i c_i, c = (0,-2,2) (2,0,-2) (-2,2,0) (0,0,2) (0,-2,0) (2,0,0)
+    const double vx = 2.0 * (al[1] - al[2] + al[5]);
+    const double vy = 2.0 * (al[2] - al[0] - al[4]);
+    const double vz = 2.0 * (al[0] - al[1] + al[3]);
+    const double inv = 1.0 / det;
+    const double bx = (d1.x * vx + d2.x * vy + d3.x * vz) * inv;
+    const double by = (d1.y * vx + d2.y * vy + d3.y * vz) * inv;
+    const double bz = (d1.z * vx + d2.z * vy + d3.z * vz) * inv;
+    int tag = tags[e];
+    // tags are validated by the host layer; clamp so a bad tag cannot read
+    // outside the material table
+    tag = tag < 1 ? 1 : (tag > ntags ? ntags : tag);
+    double nu, dnu;
+    eval_nu(bx * bx + by * by + bz * bz, tag, mat, brk, coe, nu, dnu);
+    s0[e] = make_double4(bx, by, bz, nu);
+    s1[e] = make_double2(dnu, sigma ? sigma[e] : 1.0);
+}
+
+// Loads P (or checks that the plan holds coordinates), makes sure the state
+// buffer exists and fills it.  The buffer is allocated on the first call.
+int elem_state(const mpk_plan* plan, const double* P, const int32_t* tags, const double* sol,
+               const double* sigma, const mpk_material* mat, hipStream_t st, StateView* sv) {
+    if (P) {
+        if (int rc = load_coords(plan, P, st)) return rc;
+    } else {
+        MPK_REQUIRE(plan->coords_loaded, MPK_ERR_ARGS,
+                    "P is NULL and the plan holds no coordinates (mpk_plan_set_coords)");
+    }
+    if (plan->ne == 0) return MPK_OK;
+    MPK_REQUIRE(plan->cell, MPK_ERR_ARGS, "plan without cell DOFs");
+    if (!plan->buf_state.p) MPK_HIP_TRY(plan->buf_state.alloc((size_t)plan->ne * 48));
+    double4* s0 = plan->buf_state.as<double4>();
+    double2* s1 = reinterpret_cast<double2*>(s0 + plan->ne);
+    MaterialView mv{mat->linear_nu, mat->is_nonlinear, mat->max_bsq,    mat->breaks,
+                    mat->coefs,     mat->spline_start, mat->spline_count, mat->coef_start};
+    hipLaunchKernelGGL(k_elem_state, dim3(grid_for(plan->ne, kStateBlock)), dim3(kStateBlock), 0,
+                       st, plan->T, plan->cell, plan->P4, plan->ne, sol, tags, sigma, mv,
+                       (int)mat->num_tags, (int)mat->num_breaks, (int)mat->num_coefs, s0, s1);
+    MPK_HIP_TRY(hipGetLastError());
+    *sv = StateView{s0, s1};
+    return MPK_OK;
+}
+
 template <class Op>
 int launch_exact(const mpk_plan* plan, const double* P, Op op, size_t acc_size, size_t extra,
                  hipStream_t st) {
@@ -263,30 +345,26 @@ int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
                                                       j_vals, residual, qa, 0, nullptr, nullptr},
                                 8, extra, st);
         case MPK_PREC_FP64:
+        case MPK_PREC_FP32_LOCAL: {
+            StateView sv{};
+            if (int rc = elem_state(plan, P, tags, sol_a, nullptr, mat, st, &sv)) return rc;
+            if (precision == MPK_PREC_FP64) {
+                if (calc_j)
+                    return launch_frame(plan, nullptr,
+                                        FrameJac<MPK_PREC_FP64, true>{sv, j_vals, residual, qa, 0},
+                                        8, 0, st);
+                return launch_frame(plan, nullptr,
+                                    FrameJac<MPK_PREC_FP64, false>{sv, j_vals, residual, qa, 0}, 8,
+                                    0, st);
+            }
             if (calc_j)
-                return launch_frame(plan, P,
-                                    FrameJac<MPK_PREC_FP64, true>{tags, sol_a, mv, ntags, nbrk,
-                                                                  ncoe, j_vals, residual, qa, 0,
-                                                                  nullptr, nullptr},
-                                    8, extra, st);
-            return launch_frame(plan, P,
-                                FrameJac<MPK_PREC_FP64, false>{tags, sol_a, mv, ntags, nbrk, ncoe,
-                                                               j_vals, residual, qa, 0, nullptr,
-                                                               nullptr},
-                                8, extra, st);
-        case MPK_PREC_FP32_LOCAL:
-            if (calc_j)
-                return launch_frame(plan, P,
-                                    FrameJac<MPK_PREC_FP32_LOCAL, true>{tags, sol_a, mv, ntags,
-                                                                        nbrk, ncoe, j_vals,
-                                                                        residual, qa, 0, nullptr,
-                                                                        nullptr},
-                                    8, extra, st);
-            return launch_frame(plan, P,
-                                FrameJac<MPK_PREC_FP32_LOCAL, false>{tags, sol_a, mv, ntags, nbrk,
-                                                                     ncoe, j_vals, residual, qa, 0,
-                                                                     nullptr, nullptr},
-                                8, extra, st);
+                return launch_frame(
+                    plan, nullptr,
+                    FrameJac<MPK_PREC_FP32_LOCAL, true>{sv, j_vals, residual, qa, 0}, 8, 0, st);
+            return launch_frame(plan, nullptr,
+                                FrameJac<MPK_PREC_FP32_LOCAL, false>{sv, j_vals, residual, qa, 0},
+                                8, 0, st);
+        }
     }
     set_error("unknown precision mode %d", precision);
     return MPK_ERR_ARGS;
@@ -307,24 +385,30 @@ int mpk_csc_newton_set(const mpk_plan* plan, const double* P, const int32_t* tag
     if (int rc = make_csc_quad(quad_m, false, qm)) return rc;
     MPK_REQUIRE(standard_curl_table(quad_k), MPK_ERR_ARGS,
                 "the fused Newton set needs the Nedelec P1 reference curls");
-    MaterialView mv{mat->linear_nu, mat->is_nonlinear, mat->max_bsq,    mat->breaks,
-                    mat->coefs,     mat->spline_start, mat->spline_count, mat->coef_start};
-    const int nbrk = (int)mat->num_breaks, ncoe = (int)mat->num_coefs;
-    const int ntags = (int)mat->num_tags;
-    MPK_REQUIRE(ntags >= 1, MPK_ERR_ARGS, "material table without tags");
-    const size_t extra = (nbrk + ncoe <= kMatLds) ? (size_t)(nbrk + ncoe) * 8 : 0;
+    MPK_REQUIRE(mat->num_tags >= 1, MPK_ERR_ARGS, "material table without tags");
     hipStream_t st = as_stream(stream);
+    StateView sv{};
+    if (int rc = elem_state(plan, P, tags, sol_a, sigma, mat, st, &sv)) return rc;
+    const bool with_m = m_vals != nullptr;
+    if (precision == MPK_PREC_FP64 && with_m)
+        return launch_frame(plan, nullptr,
+                            FrameNewton<MPK_PREC_FP64, true>{sv, k_vals, m_vals, j_vals, residual,
+                                                             qk, qm, 0, 0},
+                            3 * sizeof(double), 0, st);
     if (precision == MPK_PREC_FP64)
-        return launch_frame(plan, P,
-                            FrameNewton<MPK_PREC_FP64>{tags, sol_a, sigma, mv, ntags, nbrk, ncoe,
-                                                       k_vals, m_vals, j_vals, residual, qk, qm,
-                                                       0, nullptr, nullptr},
-                            sizeof(NewtonAcc), extra, st);
-    return launch_frame(plan, P,
-                        FrameNewton<MPK_PREC_FP32_LOCAL>{tags, sol_a, sigma, mv, ntags, nbrk,
-                                                         ncoe, k_vals, m_vals, j_vals, residual,
-                                                         qk, qm, 0, nullptr, nullptr},
-                        sizeof(NewtonAcc), extra, st);
+        return launch_frame(plan, nullptr,
+                            FrameNewton<MPK_PREC_FP64, false>{sv, k_vals, nullptr, j_vals,
+                                                              residual, qk, qm, 0, 0},
+                            2 * sizeof(double), 0, st);
+    if (with_m)
+        return launch_frame(plan, nullptr,
+                            FrameNewton<MPK_PREC_FP32_LOCAL, true>{sv, k_vals, m_vals, j_vals,
+                                                                   residual, qk, qm, 0, 0},
+                            3 * sizeof(double), 0, st);
+    return launch_frame(plan, nullptr,
+                        FrameNewton<MPK_PREC_FP32_LOCAL, false>{sv, k_vals, nullptr, j_vals,
+                                                                residual, qk, qm, 0, 0},
+                        2 * sizeof(double), 0, st);
 }
 
 }  // extern "C"

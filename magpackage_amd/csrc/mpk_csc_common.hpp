@@ -2,6 +2,8 @@
 // structs, small vector math and the XCD-aware tile mapping.
 #pragma once
 
+#include <type_traits>
+
 #include <cmath>
 
 #include "mpk_plan.hpp"
@@ -68,8 +70,22 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Accumulator planes of an op: ops with several outputs may keep them as
+// separate LDS planes (struct-of-arrays, plane k of tile entry i at
+// acc[i + k * size]) by declaring kPlanes; the block's plane stride is then
+// set in op.stride.  Default: one plane of Op::Acc.
+template <class Op, class = void>
+struct Planes {
+    static constexpr int value = 1;
+};
+template <class Op>
+struct Planes<Op, std::void_t<decltype(Op::kPlanes)>> {
+    static constexpr int value = Op::kPlanes;
+};
+
 // Tile prologue shared by the drivers: zeroes the block's LDS accumulator
-// range [colptr[j0], colptr[jend]) and runs the op's per-block set-up.
+// range [colptr[j0], colptr[jend]) (every plane) and runs the op's per-block
+// set-up.
 struct TileRange {
     int64_t tile_id, j0, jend;
     int base, size;
@@ -83,8 +99,9 @@ __device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* 
     t.jend = t.j0 + kW < tc.n ? t.j0 + kW : tc.n;
     t.base = tc.colptr[t.j0];
     t.size = tc.colptr[t.jend] - t.base;
+    if constexpr (Planes<Op>::value > 1) op.stride = t.size;
     op.block_init(smem + (size_t)op.acc_bytes_max);
-    for (int i = threadIdx.x; i < t.size; i += kW) acc[i] = Op::zero();
+    for (int i = threadIdx.x; i < t.size * Planes<Op>::value; i += kW) acc[i] = Op::zero();
     __syncthreads();
     return t;
 }
@@ -93,7 +110,12 @@ __device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* 
 template <class Op, class Acc>
 __device__ __forceinline__ void tile_end(const Op& op, const TileRange& t, const Acc* acc) {
     __syncthreads();
-    for (int i = threadIdx.x; i < t.size; i += kW) op.store(t.base + i, acc[i]);
+    for (int i = threadIdx.x; i < t.size; i += kW) {
+        if constexpr (Planes<Op>::value > 1)
+            op.store_planes(t.base + i, acc + i, t.size);
+        else
+            op.store(t.base + i, acc[i]);
+    }
 }
 
 }  // namespace csc

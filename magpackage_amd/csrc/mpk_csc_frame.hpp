@@ -249,90 +249,50 @@ struct FrameKM {
     }
 };
 
-// Newton Jacobian + residual.  The element's 6 edge DOFs are the column's own
-// row indices (rowind[colptr[j] + pos'_r'] == dof - 1), gathered two stages
-// ahead; the solution values of those DOFs one stage ahead.
-struct FrameRows {
-    int row[6];  // 0-based DOF of frame edge r', -1 when inactive
+// Newton quantities read the element state written by k_elem_state once per
+// call: B (the element's constant flux density, from A), nu(|B|^2), dnu and
+// sigma.  With B known, the frame curls C'_r' = 2 u_r' / det give
+// B.C'_r' = (2/det) B.u_r' directly; no DOF or solution gathers per visit.
+struct ElemState {
+    double4 s0;  // Bx, By, Bz, nu
+    double2 s1;  // dnu, sigma
 };
-struct JacElem {
-    double a[6];  // SolA of the frame edges (0 when inactive)
-    int tag;
+struct StateView {
+    const double4* __restrict__ s0;
+    const double2* __restrict__ s1;
+    __device__ ElemState load(uint32_t e) const { return ElemState{s0[e], s1[e]}; }
 };
 
+// Newton Jacobian J = K(nu) + 2 dnu (B.C_r)(B.C_c) and residual R = K(nu) a
+// (assemble_jacobian_kernel_mex.cpp:94-131 per visit, in the frame).
 template <int kPrec, bool kCalcJ>
 struct FrameJac {
     using Acc = double;
-    using Ids = FrameRows;
-    using Elem = JacElem;
+    using Ids = NoIds;
+    using Elem = ElemState;
     struct Col {
         double r;
     };
-    const int32_t* __restrict__ tags;
-    const double* __restrict__ sol;
-    MaterialView mat;
-    int ntags, nbrk, ncoe;
+    StateView st;
     double* __restrict__ out;
     double* __restrict__ res;
     CscQuad qa;
     int acc_bytes_max;
-    const double* brk_s;  // set per block (LDS or HBM)
-    const double* coe_s;
     __device__ static Acc zero() { return 0.0; }
-    __device__ void block_init(unsigned char* lds) {
-        if (nbrk + ncoe <= kMatLds) {
-            double* tab = reinterpret_cast<double*>(lds);
-            for (int i = threadIdx.x; i < nbrk; i += kW) tab[i] = mat.brk[i];
-            for (int i = threadIdx.x; i < ncoe; i += kW) tab[nbrk + i] = mat.coe[i];
-            brk_s = tab;
-            coe_s = tab + nbrk;
-        } else {
-            brk_s = mat.brk;
-            coe_s = mat.coe;
-        }
-    }
+    __device__ void block_init(unsigned char*) const {}
     __device__ Col col_begin() const { return Col{0.0}; }
     __device__ void col_end(int64_t j, Col& cs) const { res[j] = cs.r; }
     __device__ void store(int64_t p, Acc a) const {
         if (kCalcJ) out[p] = a;
     }
-    __device__ Ids ids(const TileCtx& tc, int cb, const FPos& pos) const {
-        Ids d;
-#pragma unroll
-        for (int rp = 0; rp < 6; ++rp) {
-            const int p = fpos(pos, rp);
-            d.row[rp] = (p != 0xff) ? tc.rowind[cb + p] : -1;
-        }
-        return d;
-    }
-    __device__ Elem elem(uint32_t e, const Ids& d) const {
-        Elem el;
-#pragma unroll
-        for (int rp = 0; rp < 6; ++rp) el.a[rp] = d.row[rp] >= 0 ? sol[d.row[rp]] : 0.0;
-        const int tag = tags[e];
-        // tags are validated by the host layer; clamp so a bad tag cannot read
-        // outside the material table
-        el.tag = tag < 1 ? 1 : (tag > ntags ? ntags : tag);
-        return el;
-    }
+    __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
+    __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
     __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
                             Acc* acc, Col& cs) const {
         const FrameGeom g = frame_geom(nodes, false);
-        // C'_r' = 2 u_r' / det;  B = sum a'_r' C'_r' with a' = A * f
+        const Vec3 Bv{el.s0.x, el.s0.y, el.s0.z};
+        const double nu = el.s0.w, dnu = el.s1.x;
         const double two_inv = 2.0 / g.det;
-        Vec3 Bv{0.0, 0.0, 0.0};
-#pragma unroll
-        for (int rp = 0; rp < 6; ++rp) {
-            const double ap = el.a[rp] * fsign(f, rp);
-            Bv.x += ap * g.u[rp].x;
-            Bv.y += ap * g.u[rp].y;
-            Bv.z += ap * g.u[rp].z;
-        }
-        Bv.x *= two_inv;
-        Bv.y *= two_inv;
-        Bv.z *= two_inv;
-        double nu, dnu;
-        eval_nu(dot(Bv, Bv), el.tag, mat, brk_s, coe_s, nu, dnu);
         // w|det| summed over the rule; C'.C' = 4 u.u / det^2
         const double wd = qa.wsum * fabs(g.det);
         const double f0 = fsign(f, 0);
@@ -353,118 +313,80 @@ struct FrameJac {
 };
 
 // Fused Newton set: in one pass over the plan, the secant stiffness K(nu(B))
-// (nu from the B-H table at the element's B, the K the reference assembles
-// with assembleStiffness from those per-element nu), the sigma-mass M, the
-// Jacobian J = K(nu) + 2 dnu (B.C_r)(B.C_c) and the residual R = K(nu) a.
-// Outputs may be NULL (skipped).  The curl-curl quadrature weight sum wk and
-// the mass moments come from the two tables the separate calls would use.
-struct NewtonAcc {
-    double k, m, j;
-};
-template <int kPrec>
+// (the K assembleStiffness builds from the per-element nu the B-H curve gives
+// at B), the sigma-mass M, the Jacobian J and the residual R.  Outputs may be
+// NULL (skipped).  LDS planes K, D = J - K (the 2 dnu (B.C)(B.C) term) and,
+// when kWithM, M: visits of linear elements (dnu = 0) touch only K and visits
+// of non-conducting ones (sigma = 0) skip M; J = K + D at the store.  Without
+// M the tile needs 16 B of LDS per entry instead of 24 (more waves per CU).
+template <int kPrec, bool kWithM>
 struct FrameNewton {
-    using Acc = NewtonAcc;
-    using Ids = FrameRows;
-    struct Elem {
-        JacElem jac;
-        double sigma;
-    };
+    using Acc = double;
+    using Ids = NoIds;
+    using Elem = ElemState;
     struct Col {
         double r;
     };
-    const int32_t* __restrict__ tags;
-    const double* __restrict__ sol;
-    const double* __restrict__ sigma;  // NULL: sigma = 1
-    MaterialView mat;
-    int ntags, nbrk, ncoe;
+    static constexpr int kPlanes = kWithM ? 3 : 2;
+    static constexpr int kD = kWithM ? 2 : 1;  // plane of D
+    StateView st;
     double* __restrict__ out_k;
     double* __restrict__ out_m;
     double* __restrict__ out_j;
     double* __restrict__ res;
     CscQuad qk, qm;
     int acc_bytes_max;
-    const double* brk_s;
-    const double* coe_s;
-    __device__ static Acc zero() { return NewtonAcc{0.0, 0.0, 0.0}; }
-    __device__ void block_init(unsigned char* lds) {
-        if (nbrk + ncoe <= kMatLds) {
-            double* tab = reinterpret_cast<double*>(lds);
-            for (int i = threadIdx.x; i < nbrk; i += kW) tab[i] = mat.brk[i];
-            for (int i = threadIdx.x; i < ncoe; i += kW) tab[nbrk + i] = mat.coe[i];
-            brk_s = tab;
-            coe_s = tab + nbrk;
-        } else {
-            brk_s = mat.brk;
-            coe_s = mat.coe;
-        }
-    }
+    int stride;  // plane stride (the block's tile size), set by tile_begin
+    __device__ static Acc zero() { return 0.0; }
+    __device__ void block_init(unsigned char*) const {}
     __device__ Col col_begin() const { return Col{0.0}; }
     __device__ void col_end(int64_t j, Col& cs) const {
         if (res) res[j] = cs.r;
     }
-    __device__ void store(int64_t p, Acc a) const {
-        if (out_k) out_k[p] = a.k;
-        if (out_m) out_m[p] = a.m;
-        if (out_j) out_j[p] = a.j;
+    __device__ void store_planes(int64_t p, const Acc* a, int size) const {
+        const double k = a[0];
+        if (out_k) out_k[p] = k;
+        if (kWithM) out_m[p] = a[size];
+        if (out_j) out_j[p] = k + a[kD * size];
     }
-    __device__ Ids ids(const TileCtx& tc, int cb, const FPos& pos) const {
-        Ids d;
-#pragma unroll
-        for (int rp = 0; rp < 6; ++rp) {
-            const int p = fpos(pos, rp);
-            d.row[rp] = (p != 0xff) ? tc.rowind[cb + p] : -1;
-        }
-        return d;
-    }
-    __device__ Elem elem(uint32_t e, const Ids& d) const {
-        Elem el;
-#pragma unroll
-        for (int rp = 0; rp < 6; ++rp) el.jac.a[rp] = d.row[rp] >= 0 ? sol[d.row[rp]] : 0.0;
-        const int tag = tags[e];
-        el.jac.tag = tag < 1 ? 1 : (tag > ntags ? ntags : tag);
-        el.sigma = sigma ? sigma[e] : 1.0;
-        return el;
-    }
+    __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
+    __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
     __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
                             Acc* acc, Col& cs) const {
-        const bool has_m = fabs(el.sigma) > 1e-12;
+        const double sigma = el.s1.y;
+        const bool has_m = kWithM && fabs(sigma) > 1e-12;
         const FrameGeom g = frame_geom(nodes, has_m);
+        const Vec3 Bv{el.s0.x, el.s0.y, el.s0.z};
+        const double nu = el.s0.w, dnu = el.s1.x;
         const double two_inv = 2.0 / g.det;
-        Vec3 Bv{0.0, 0.0, 0.0};
-#pragma unroll
-        for (int rp = 0; rp < 6; ++rp) {
-            const double ap = el.jac.a[rp] * fsign(f, rp);
-            Bv.x += ap * g.u[rp].x;
-            Bv.y += ap * g.u[rp].y;
-            Bv.z += ap * g.u[rp].z;
-        }
-        Bv.x *= two_inv;
-        Bv.y *= two_inv;
-        Bv.z *= two_inv;
-        double nu, dnu;
-        eval_nu(dot(Bv, Bv), el.jac.tag, mat, brk_s, coe_s, nu, dnu);
         const double inv = 1.0 / fabs(g.det);
         const double wd = qk.wsum * fabs(g.det);
         const double f0 = fsign(f, 0);
         const double BC0 = two_inv * dot(Bv, g.u[0]);
         cs.r += nu * BC0 * wd * f0;
         const double ks = 4.0 * qk.wsum * nu * inv * f0;  // = nu * (4/det^2) * wd
-        const double ms = el.sigma * inv * f0;
-        double m[6] = {0, 0, 0, 0, 0, 0};
-        if (has_m && out_m) mass_col<kPrec>(g, qm, m);
 #pragma unroll
         for (int rp = 0; rp < 6; ++rp) {
             const int p = fpos(pos, rp);
-            const double s = fsign(f, rp);
-            const double kv = curl_dot<kPrec>(g, rp) * (ks * s);
-            const double BCr = two_inv * dot(Bv, g.u[rp]);
-            const double jv = kv + 2.0 * dnu * BCr * BC0 * wd * (s * f0);
-            if (p != 0xff) {
-                Acc a = acc[p];
-                a.k += kv;
-                a.m += m[rp] * (ms * s);
-                a.j += jv;
-                acc[p] = a;
+            if (p != 0xff) acc[p] += curl_dot<kPrec>(g, rp) * (ks * fsign(f, rp));
+        }
+        if (has_m) {
+            const double ms = sigma * inv * f0;
+            double m[6];
+            mass_col<kPrec>(g, qm, m);
+#pragma unroll
+            for (int rp = 0; rp < 6; ++rp) {
+                const int p = fpos(pos, rp);
+                if (p != 0xff) acc[stride + p] += m[rp] * (ms * fsign(f, rp));
+            }
+        }
+        if (out_j && dnu != 0.0) {
+            const double js = 2.0 * dnu * BC0 * wd * f0;
+#pragma unroll
+            for (int rp = 0; rp < 6; ++rp) {
+                const int p = fpos(pos, rp);
+                if (p != 0xff)
+                    acc[kD * stride + p] += (two_inv * dot(Bv, g.u[rp])) * (js * fsign(f, rp));
             }
         }
     }

@@ -22,7 +22,12 @@ struct mpk_plan {
     uint32_t* inc_slot = nullptr; // [ninc], 6*e + c
     uint2* inc_pos = nullptr;     // [ninc], byte r = in-column position of row r (0xFF: none)
     int4* T = nullptr;            // [ne], 0-based node ids
-    mpk::DevBuf buf_colptr, buf_rowind, buf_incptr, buf_incslot, buf_incpos, buf_T;
+    int32_t* cell = nullptr;      // [6*ne], 0-based DOF of local edge c (-1: inactive)
+    mpk::DevBuf buf_colptr, buf_rowind, buf_incptr, buf_incslot, buf_incpos, buf_T, buf_cell;
+    // Element state of the Newton kernels, [ne] x 48 B, allocated on first use:
+    // double4 (Bx, By, Bz, nu) at state, then double2 (dnu/d|B|^2, sigma) at
+    // state + 4*ne.
+    mutable mpk::DevBuf buf_state;
 
     // Closed-form (frame) kernels: one 16-byte record per incidence.
     //   x = element id

@@ -229,6 +229,14 @@ __global__ void k_plan_T(const int32_t* __restrict__ T, int64_t ne, int64_t nn,
     T4[e] = t;
 }
 
+// 0-based copy of the cell DOFs (-1: inactive); values outside [0, n] are
+// caught by k_plan_keys before any kernel reads this copy.
+__global__ void k_plan_cell(const int32_t* __restrict__ CD, int64_t ne, int32_t* __restrict__ cell) {
+    const int64_t s = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (s >= 6 * ne) return;
+    cell[s] = CD[s] - 1;
+}
+
 __global__ void k_plan_keys(const int32_t* __restrict__ CD, int64_t ne, int64_t n,
                             uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                             int* __restrict__ err) {
@@ -728,6 +736,13 @@ int mpk_plan_create(const int32_t* T, const int32_t* cell_dofs, int64_t ne, int6
     }
     hipLaunchKernelGGL(k_plan_T, dim3(grid_for(ne, kB)), dim3(kB), 0, st, T, ne, nn, plan->T,
                        err.as<int>());
+    if (plan->buf_cell.alloc(6 * ne * sizeof(int32_t)) != hipSuccess) {
+        set_error("hipMalloc failed for the plan's cell DOFs");
+        return fail(MPK_ERR_HIP);
+    }
+    plan->cell = plan->buf_cell.as<int32_t>();
+    hipLaunchKernelGGL(k_plan_cell, dim3(grid_for(6 * ne, kB)), dim3(kB), 0, st, cell_dofs, ne,
+                       plan->cell);
     {
         DevBuf k0, k1, v0, v1, head, seg, nv, segstart, colcount, inccount, incidx, posb;
         auto A = [&](DevBuf& b, size_t bytes) { return b.alloc(bytes) == hipSuccess; };
