@@ -213,6 +213,106 @@ def run_newton(args):
     }), flush=True)
 
 
+def run_ecore(args):
+    """Nonlinear transient on the synthetic E-core: plain Newton (x_prev start,
+    the reference's choice) against the learned-reluctivity warm start, same
+    mesh, source, tolerances and linear solver; the network is trained on the
+    first --train-steps steps of the learned run itself."""
+    import torch
+    from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
+    from magpackage_amd.boundary import find_outer_boundary_dofs
+    from magpackage_amd.learned import NuPredictor, learned_warm_start
+    from magpackage_amd.material import create_linear, marrocco
+    from magpackage_amd.meshgen import ecore_box
+    from magpackage_amd.transient import TransientSolver
+
+    torch.cuda.set_device(0)
+    ec = ecore_box(*args.ecore_cubes)
+    mesh = Mesh.from_kuhn(ec.mesh)
+    dh = DofHandler(mesh)
+    space = FunctionSpace("Nedelec", 1)
+    dh.distributeDofs(space)
+    asm = Assembler(mesh, dh)
+    fixed = find_outer_boundary_dofs(mesh, dh, space)
+    lib = {1: create_linear(1.0), 2: marrocco(), 3: create_linear(1.0)}
+    sigma = {2: 2e6}
+    js = torch.from_numpy(ec.js_dir).cuda()
+    amp, freq, dt = 3e6, 50.0, args.dt
+    src = lambda t: js * (amp * np.sin(2 * np.pi * freq * t))
+    steps = [dt] * args.steps
+    ne, n = ec.mesh.num_elems, dh.NumGlobalDofs
+    log(f"[ecore] {ne} tets, {n} edge DOFs, {int(fixed.sum())} fixed; {args.steps} steps of "
+        f"{dt:g} s, train on steps 1-{args.train_steps}")
+
+    def run(kind: str):
+        solver = TransientSolver(asm)
+        solver.Verbose = True
+        hook, on_step, pred = None, None, None
+        if kind == "extrapolated":
+            solver.InitialGuess = "extrapolated"
+        if kind == "learned":
+            pred = NuPredictor(asm, lib, seed=0)
+            hook = learned_warm_start(pred, space)
+            solver.InitialGuess = "extrapolated"  # extrapolation stays the fallback
+            bcache = {}
+
+            def on_step(ctx):
+                k = ctx["step"]
+                if k > args.train_steps:
+                    return
+                bp = bcache.get(k - 1)
+                if bp is None:
+                    bp = asm.computeElementB(ctx["x_prev"], space)
+                bp2 = bcache.get(k - 2, bp)
+                bx = asm.computeElementB(ctx["x"], space)
+                bcache[k] = bx
+                bcache.pop(k - 3, None)
+                pred.add_sample(bp, bp2, bx)
+                if k == args.train_steps:
+                    info = pred.fit(iters=args.train_iters)
+                    log(f"[ecore] trained: {info}")
+        t0 = time.perf_counter()
+        _, infos = solver.solve(space, lib, sigma, src, steps, fixed, warm_start=hook,
+                                on_step=on_step)
+        torch.cuda.synchronize()
+        return infos, time.perf_counter() - t0, pred
+
+    k0 = args.train_steps
+    ev = lambda infos, key: float(np.mean([getattr(i, key) for i in infos[k0:]]))
+    res = {"eval_steps": f"{k0 + 1}-{args.steps}"}
+    for kind in ("plain", "extrapolated", "learned"):
+        if kind == "extrapolated" and args.no_extrap_run:
+            continue
+        log(f"[ecore] {kind}")
+        infos, total, pred = run(kind)
+        res[kind] = {"ms_per_step": ev(infos, "wall_ms"), "newton": ev(infos, "newton_iters"),
+                     "pcg": ev(infos, "pcg_iters"), "total_s": total,
+                     "converged": all(i.converged for i in infos),
+                     "newton_per_step": [i.newton_iters for i in infos],
+                     "ms_per_step_all": [round(i.wall_ms, 1) for i in infos]}
+        if pred is not None:
+            res[kind]["warm_used"] = float(np.mean([i.warm_start == "warm" for i in infos[k0:]]))
+            res[kind]["train"] = pred.train_info
+    plain, learn = res["plain"], res["learned"]
+    speed = plain["ms_per_step"] / learn["ms_per_step"]
+    for kind in ("plain", "extrapolated", "learned"):
+        if kind in res:
+            r = res[kind]
+            log(f"[ecore] steps {k0 + 1}-{args.steps} {kind}: {r['ms_per_step']:.1f} ms/step, "
+                f"{r['newton']:.2f} Newton, {r['pcg']:.0f} PCG iterations")
+    log(f"[ecore] learned vs plain: {speed:.2f}x")
+    print(json.dumps({
+        "metric": METRIC, "value": speed, "unit": "x (plain / learned time per step)",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": learn["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (MLP bf16/f32)",
+        "data": "synthetic E-core (jittered Kuhn box), Marrocco iron, 50 Hz winding",
+        "config": {"workload": "ecore_transient_learned_vs_plain", "tets": ne, "edge_dofs": n,
+                   "dt": dt, "train_steps": k0, "rel_tol": 1e-6},
+        "nonlinear": res,
+    }), flush=True)
+
+
 def timed_steps(step, stream, steps, warmup):
     """Runs warmup + timed steps; returns (ms per step over the timed region,
     mean ms of the bracketed kernel launches)."""
@@ -298,10 +398,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="freq128", choices=["freq128", "newton4m"],
+    ap.add_argument("--workload", default="freq128", choices=["freq128", "newton4m", "ecore"],
                     help="freq128: K + 1j*w*M_sigma complex128 on the 128^3 box (default); "
                          "newton4m: fused K(nu(B)) + M_sigma + J + R on the 100x100x67 box")
     ap.add_argument("--n", type=int, default=128, help="cubes per axis of the Kuhn box")
+    ap.add_argument("--dt", type=float, default=1e-4, help="ecore: time step (s)")
+    ap.add_argument("--train-steps", type=int, default=40, help="ecore: steps used for training")
+    ap.add_argument("--train-iters", type=int, default=1500, help="ecore: Adam iterations")
+    ap.add_argument("--ecore-cubes", type=int, nargs=3, default=[64, 64, 40],
+                    help="ecore: cubes per axis (64 64 40: 983,040 tets)")
+    ap.add_argument("--no-extrap-run", action="store_true",
+                    help="ecore: skip the plain-Newton-from-extrapolation reference run")
     ap.add_argument("--split-m", action="store_true",
                     help="newton4m: K, J, R in one pass and M_sigma in a second one")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp64_exact", "fp32_local"],
@@ -319,6 +426,10 @@ def main():
         print(json.dumps({"error": "this engine is measured on one GPU only; "
                                    "run with --gpus 1 (multi-GPU sharding is a design note)"}))
         sys.exit(2)
+    if args.workload == "ecore":
+        if args.steps == 20:
+            args.steps = 200
+        return run_ecore(args)
     if args.workload == "newton4m":
         args.precision = args.precision or "fp32_local"
         if args.precision == "fp64_exact":

@@ -128,6 +128,12 @@ int mpk_element_b(const double* P, int64_t nn, const int32_t* T, const int32_t* 
                   int64_t ne, const double* sol, int64_t num_sol, const mpk_quadrature* quad,
                   double* B_out, void* stream);
 
+/* evaluate_nu_derivative (src/mex/MexMaterialUtils.hpp:24-87) at n points:
+ * s [n] (|B|^2) and tags [n] (1-based int32) device; nu_out [n], dnu_out [n]
+ * (may be NULL).  Synchronises `stream` for the tag check. */
+int mpk_eval_nu(const double* s, const int32_t* tags, int64_t n, const mpk_material* mat,
+                double* nu_out, double* dnu_out, void* stream);
+
 /* ========================================================================== */
 /* 2. COO -> CSC with MATLAB sparse() semantics                               */
 /* ========================================================================== */

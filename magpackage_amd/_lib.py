@@ -1,4 +1,5 @@
-"""ctypes binding of libmpk_assembly.so (the C ABI in include/mpk_assembly.h).
+"""ctypes binding of libmpk_assembly.so (the C ABI in include/mpk_assembly.h and
+include/mpk_mlp.h).
 
 The library is required: importing this module raises ``ImportError`` when
 the built ``libmpk_assembly.so`` is missing, and there is no CPU fallback.
@@ -73,6 +74,7 @@ _SIGS = {
     "mpk_source_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _pq, _vp, _vp, _vp], _i32),
     "mpk_element_b": ([_vp, _i64, _vp, _vp, _i64, _vp, _i64, _pq, _vp, _vp], _i32),
     "mpk_csc_source": ([_plan_p, _vp, _vp, _pq, _vp, _vp], _i32),
+    "mpk_eval_nu": ([_vp, _vp, _i64, _pm, _vp, _vp, _vp], _i32),
     "mpk_sparse_coo_to_csc": ([_vp, _vp, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _i64,
                                _pi64, _vp], _i32),
     "mpk_generate_edges": ([_vp, _i64, _i64, _vp, _vp, _vp, _pi64, _vp], _i32),
@@ -94,6 +96,12 @@ _SIGS = {
     "mpk_spmv_sym": ([_vp, _vp, _vp, _i64, _vp, _vp, _vp], _i32),
     "mpk_csc_combine": ([_i64, _d, _vp, _d, _vp, _d, _vp, _d, _vp, _vp, _vp], _i32),
     "mpk_pcg_work_size": ([_i64, _pi64], _i32),
+    "mpk_mlp_sync_bf16": ([_vp, _vp, _vp], _i32),
+    "mpk_mlp_forward": ([_vp, _vp, _vp, _i64, _vp, _vp], _i32),
+    "mpk_mlp_work_size": ([_i64, _pi64], _i32),
+    "mpk_mlp_loss_grad": ([_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp], _i32),
+    "mpk_mlp_adam": ([_vp, _vp, _vp, _vp, _i64, C.c_float, C.c_float, C.c_float, C.c_float,
+                      _vp, _vp], _i32),
     "mpk_pcg_solve": ([_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _d, _i32, _i32,
                        C.POINTER(C.c_int), C.POINTER(C.c_double), _vp], _i32),
 }

@@ -354,7 +354,7 @@ int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
                                         FrameJac<MPK_PREC_FP64, true>{sv, j_vals, residual, qa, 0},
                                         8, 0, st);
                 return launch_frame(plan, nullptr,
-                                    FrameJac<MPK_PREC_FP64, false>{sv, j_vals, residual, qa, 0}, 8,
+                                    FrameJac<MPK_PREC_FP64, false>{sv, j_vals, residual, qa, 0}, 0,
                                     0, st);
             }
             if (calc_j)
@@ -363,7 +363,7 @@ int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
                     FrameJac<MPK_PREC_FP32_LOCAL, true>{sv, j_vals, residual, qa, 0}, 8, 0, st);
             return launch_frame(plan, nullptr,
                                 FrameJac<MPK_PREC_FP32_LOCAL, false>{sv, j_vals, residual, qa, 0},
-                                8, 0, st);
+                                0, 0, st);
         }
     }
     set_error("unknown precision mode %d", precision);

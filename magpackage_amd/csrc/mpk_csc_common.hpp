@@ -73,7 +73,8 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 // Accumulator planes of an op: ops with several outputs may keep them as
 // separate LDS planes (struct-of-arrays, plane k of tile entry i at
 // acc[i + k * size]) by declaring kPlanes; the block's plane stride is then
-// set in op.stride.  Default: one plane of Op::Acc.
+// set in op.stride.  kPlanes = 0: the op writes no matrix values (no LDS
+// accumulator, nothing stored per tile).  Default: one plane of Op::Acc.
 template <class Op, class = void>
 struct Planes {
     static constexpr int value = 1;
@@ -102,13 +103,14 @@ __device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* 
     if constexpr (Planes<Op>::value > 1) op.stride = t.size;
     op.block_init(smem + (size_t)op.acc_bytes_max);
     for (int i = threadIdx.x; i < t.size * Planes<Op>::value; i += kW) acc[i] = Op::zero();
-    __syncthreads();
+    if constexpr (Planes<Op>::value > 0) __syncthreads();
     return t;
 }
 
 // Tile epilogue: the finished accumulator range leaves as contiguous stores.
 template <class Op, class Acc>
 __device__ __forceinline__ void tile_end(const Op& op, const TileRange& t, const Acc* acc) {
+    if constexpr (Planes<Op>::value == 0) return;
     __syncthreads();
     for (int i = threadIdx.x; i < t.size; i += kW) {
         if constexpr (Planes<Op>::value > 1)

@@ -273,6 +273,7 @@ struct FrameJac {
     struct Col {
         double r;
     };
+    static constexpr int kPlanes = kCalcJ ? 1 : 0;  // residual only: no LDS tile
     StateView st;
     double* __restrict__ out;
     double* __restrict__ res;

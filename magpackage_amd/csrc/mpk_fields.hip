@@ -173,6 +173,25 @@ k_element_b(const double* __restrict__ P, int64_t nn, const int32_t* __restrict_
     B[3 * e + 2] = b2;
 }
 
+// (nu, dnu/ds) = evaluate_nu_derivative(s, tag) (MexMaterialUtils.hpp:24-87)
+// for a batch of points, tables read through the cache.
+__global__ void __launch_bounds__(kB)
+k_eval_nu(const double* __restrict__ s, const int32_t* __restrict__ tags, int64_t n,
+          MaterialView mat, int ntags, double* __restrict__ nu, double* __restrict__ dnu,
+          int* __restrict__ err) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    int tag = tags[i];
+    if (tag < 1 || tag > ntags) {
+        atomicOr(err, 1);
+        tag = 1;
+    }
+    double a, b;
+    eval_nu(s[i], tag, mat, mat.brk, mat.coe, a, b);
+    nu[i] = a;
+    if (dnu) dnu[i] = b;
+}
+
 ValArg make_arg(const mpk_quadrature* q, bool curl) {
     ValArg a;
     std::memset(&a, 0, sizeof(a));
@@ -251,6 +270,23 @@ int mpk_element_b(const double* P, int64_t nn, const int32_t* T, const int32_t* 
                        cell_dofs, ne, sol, num_sol, make_arg(quad, true), B_out,
                        err.as<int>());
     return check_err(st, err, "node or DOF index outside its range");
+}
+
+int mpk_eval_nu(const double* s, const int32_t* tags, int64_t n, const mpk_material* mat,
+                double* nu_out, double* dnu_out, void* stream) {
+    clear_error();
+    MPK_REQUIRE(mat && n >= 0, MPK_ERR_ARGS, "bad argument");
+    if (n == 0) return MPK_OK;
+    MPK_REQUIRE(s && tags && nu_out && mat->num_tags >= 1, MPK_ERR_ARGS, "NULL argument");
+    MaterialView mv{mat->linear_nu, mat->is_nonlinear, mat->max_bsq,    mat->breaks,
+                    mat->coefs,     mat->spline_start, mat->spline_count, mat->coef_start};
+    hipStream_t st = as_stream(stream);
+    DevBuf err;
+    MPK_HIP_TRY(err.alloc(sizeof(int)));
+    MPK_HIP_TRY(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_eval_nu, dim3(grid_for(n, kB)), dim3(kB), 0, st, s, tags, n, mv,
+                       (int)mat->num_tags, nu_out, dnu_out, err.as<int>());
+    return check_err(st, err, "region tag outside the material table");
 }
 
 }  // extern "C"

@@ -1,0 +1,210 @@
+"""Learned reluctivity predictor: warm start for the transient Newton solve.
+
+No reference counterpart (MagPackage has no learned component).  For every
+element of a nonlinear (B-H) region, a small MLP (include/mpk_mlp.h: 16 ->
+64 -> 64 -> 64 -> 1, tanh, BF16 weights on the matrix cores, FP32
+accumulation) maps
+
+    (B_prev, B_extrap = 2 B_prev - B_prev2, |B_prev|^2, |B_extrap|^2, region)
+
+to the log-reluctivity the converged solution of the next step will have.
+The warm start assembles K(nu_hat) + M_sigma/dt + reg M_reg (the Newton
+system's secant form, whose solution is the converged field when nu_hat is
+exact), solves it once, and hands the result to plain Newton; the driver
+keeps x_prev instead when the warm start's residual is not smaller.  The
+network is trained on the GPU, by the project's own forward / backward /
+Adam kernels, on the (features, converged nu) pairs of the first steps of
+the same run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .material import MaterialData
+
+B_SCALE = 2.2     # T: feature scale of B
+S_SCALE = 4.84    # T^2: feature scale of |B|^2
+N_IN = 16
+NPARAM = 9473
+NBF16 = 9216
+
+
+def _glorot(seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_out, fan_in, bias in ((64, 16, True), (64, 64, True), (64, 64, True)):
+        lim = math.sqrt(6.0 / (fan_in + fan_out))
+        parts += [rng.uniform(-lim, lim, fan_out * fan_in), np.zeros(fan_out)]
+    lim = math.sqrt(6.0 / 65.0)
+    parts += [rng.uniform(-lim, lim, 64), np.zeros(1)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == NPARAM
+    return p
+
+
+class NuPredictor:
+    """Per-element reluctivity predictor for the elements of nonlinear regions."""
+
+    def __init__(self, assembler, matLib: dict, seed: int = 0):
+        self.asm = assembler
+        mesh = assembler.Mesh
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        nonlin = sorted(int(k) for k, v in matLib.items() if isinstance(v, MaterialData)
+                        and v.Type == "Nonlinear")
+        if not nonlin:
+            raise ValueError("no nonlinear material: nothing to predict")
+        if len(nonlin) > N_IN - 8:
+            raise ValueError("too many nonlinear regions for the feature vector")
+        self.tags = nonlin
+        tags = mesh.RegionTags
+        sel = np.isin(tags, nonlin)
+        self.idx = torch.from_numpy(np.nonzero(sel)[0].astype(np.int64)).to(self.dev)
+        onehot = np.zeros((int(sel.sum()), len(nonlin)), dtype=np.float32)
+        for j, t in enumerate(nonlin):
+            onehot[tags[sel] == t, j] = 1.0
+        self.onehot = torch.from_numpy(onehot).to(self.dev)
+        self.tags_sel = torch.from_numpy(tags[sel].astype(np.int32)).to(self.dev)
+        self.mat, self._keep = assembler.material_tables(matLib)
+        # reluctivity of the linear regions' elements (their constant nu)
+        nu0 = np.zeros(int(tags.max()) + 1)
+        for k, v in matLib.items():
+            if isinstance(v, MaterialData):
+                nu0[int(k)] = v.Nu_Linear if v.Type == "Linear" else v.nu0
+            else:
+                nu0[int(k)] = float(v)
+        self.nu_lin = torch.from_numpy(nu0[tags]).to(self.dev)
+        self.params = torch.from_numpy(_glorot(seed)).to(self.dev)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.wb = torch.empty(NBF16, dtype=torch.int16, device=self.dev)
+        L.check(L.lib.mpk_mlp_sync_bf16(L.ptr(self.params), L.ptr(self.wb), L.stream_ptr()))
+        self.X, self.T = [], []
+        self.t_mean, self.t_std = 0.0, 1.0
+        self.adam_step = 0
+        self.trained = False
+        self.train_info = {}
+
+    # ------------------------------------------------------------- features
+    def features(self, B_prev: torch.Tensor, B_prev2: torch.Tensor) -> torch.Tensor:
+        bp = B_prev.index_select(0, self.idx)
+        be = 2.0 * bp - B_prev2.index_select(0, self.idx)
+        X = torch.zeros((self.idx.numel(), N_IN), dtype=torch.float32, device=self.dev)
+        X[:, 0:3] = (bp / B_SCALE).float()
+        X[:, 3:6] = (be / B_SCALE).float()
+        X[:, 6] = ((bp * bp).sum(dim=1) / S_SCALE).float()
+        X[:, 7] = ((be * be).sum(dim=1) / S_SCALE).float()
+        X[:, 8:8 + self.onehot.shape[1]] = self.onehot
+        return X
+
+    def log_nu(self, B: torch.Tensor) -> torch.Tensor:
+        """log nu(|B|^2) of the selected elements from the B-H tables."""
+        s = (B.index_select(0, self.idx) ** 2).sum(dim=1).contiguous()
+        nu = torch.empty_like(s)
+        L.check(L.lib.mpk_eval_nu(L.ptr(s), L.ptr(self.tags_sel), s.numel(), C.byref(self.mat),
+                                  L.ptr(nu), None, L.stream_ptr()))
+        return torch.log(nu)
+
+    def add_sample(self, B_prev, B_prev2, B_conv):
+        self.X.append(self.features(B_prev, B_prev2))
+        self.T.append(self.log_nu(B_conv).float())
+
+    # ------------------------------------------------------------- training
+    def fit(self, iters: int = 1500, batch: int = 8192, lr: float = 2e-3, seed: int = 0):
+        X = torch.cat(self.X)
+        T = torch.cat(self.T)
+        self.t_mean = float(T.mean())
+        self.t_std = float(T.std()) or 1.0
+        Tn = ((T - self.t_mean) / self.t_std).contiguous()
+        n = X.shape[0]
+        b = min(batch, n)
+        ws = C.c_int64()
+        L.check(L.lib.mpk_mlp_work_size(b, C.byref(ws)))
+        work = torch.empty(ws.value, dtype=torch.float32, device=self.dev)
+        g = torch.empty(NPARAM, dtype=torch.float32, device=self.dev)
+        loss = torch.empty(1, dtype=torch.float32, device=self.dev)
+        gen = torch.Generator(device=self.dev)
+        gen.manual_seed(seed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        first = None
+        for it in range(iters):
+            sel = torch.randint(0, n, (b,), device=self.dev, generator=gen)
+            xb = X.index_select(0, sel).contiguous()
+            tb = Tn.index_select(0, sel).contiguous()
+            L.check(L.lib.mpk_mlp_loss_grad(L.ptr(self.params), L.ptr(self.wb), L.ptr(xb),
+                                            L.ptr(tb), b, L.ptr(g), L.ptr(loss), L.ptr(work),
+                                            L.stream_ptr()))
+            if first is None:
+                first = float(loss)
+            self.adam_step += 1
+            step_lr = lr * (0.1 ** (it / max(iters, 1)))
+            L.check(L.lib.mpk_mlp_adam(L.ptr(self.params), L.ptr(self.m), L.ptr(self.v), L.ptr(g),
+                                       self.adam_step, step_lr, 0.9, 0.999, 1e-8, L.ptr(self.wb),
+                                       L.stream_ptr()))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        # full-set loss (normalised units) after training
+        yhat = self._forward(X)
+        mse = float(((yhat - Tn) ** 2).mean())
+        self.trained = True
+        self.train_info = dict(samples=n, iters=iters, batch=b, first_loss=first, mse=mse,
+                               rms_log_nu=math.sqrt(mse) * self.t_std,
+                               train_ms=1e3 * (t1 - t0))
+        return self.train_info
+
+    def _forward(self, X: torch.Tensor) -> torch.Tensor:
+        y = torch.empty(X.shape[0], dtype=torch.float32, device=self.dev)
+        L.check(L.lib.mpk_mlp_forward(L.ptr(self.params), L.ptr(self.wb), L.ptr(X.contiguous()),
+                                      X.shape[0], L.ptr(y), L.stream_ptr()))
+        return y
+
+    def predict_nu(self, B_prev, B_prev2) -> torch.Tensor:
+        """nu_hat for every element: the prediction in nonlinear regions, nu0 elsewhere."""
+        y = self._forward(self.features(B_prev, B_prev2))
+        nu = self.nu_lin.clone()
+        nu[self.idx] = torch.exp(y.double() * self.t_std + self.t_mean)
+        return nu
+
+
+def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5):
+    """warm_start hook for TransientSolver.solve (None until the network is
+    trained).  With nu_hat predicted from B_prev and its extrapolation, it
+    solves the secant system A x = b, A = K(nu_hat) + M_sigma/dt + reg M_reg,
+    for the correction around the extrapolated state x_e = 2 x_prev - x_prev2:
+    A d = b - A x_e, x0 = x_e + d, to the step's Newton target
+    (eta_floor * max(Tolerance, RelTolerance |R(x_prev)|) absolute), so that
+    when nu_hat is exact x0 already meets the convergence test."""
+    asm = predictor.asm
+
+    def hook(ctx):
+        if not predictor.trained:
+            return None
+        s = ctx["solver"]
+        x_prev, x_prev2, b, dt = ctx["x_prev"], ctx["x_prev2"], ctx["b"], ctx["dt"]
+        bp = asm.computeElementB(x_prev, space)
+        bp2 = asm.computeElementB(x_prev2, space)
+        nu = predictor.predict_nu(bp, bp2).contiguous()
+        kv = torch.empty_like(s.jsys)
+        L.check(L.lib.mpk_csc_curl_curl(s.plan.handle, None, L.ptr(nu), C.byref(s.qk),
+                                        L.PREC_FP64, L.ptr(kv), s.sp))
+        s._constant_part(dt)
+        s._combine(kv, 1.0, kv, 1.0, s.cvals)
+        xe = 2.0 * x_prev - x_prev2 if ctx["step"] > 2 else x_prev
+        s._spmv(kv, xe, s.tmp)
+        r = (b - s.tmp) * s.free
+        rn = float(torch.linalg.vector_norm(r))
+        target = max(s.Tolerance, s.RelTolerance * ctx["r_prev"])
+        if rn <= eta_floor * target:
+            return xe
+        eta = min(max(eta_floor * target / rn, 1e-12), 1e-1)
+        d, nit = s.solve_linear_system(kv, r, eta)
+        ctx["warm_pcg"] = nit
+        return xe + d
+
+    return hook

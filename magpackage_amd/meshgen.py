@@ -135,3 +135,45 @@ def tag_box(mesh: KuhnMesh, lo, hi, tag: int) -> None:
         c = centroid_axis(mesh, ax)
         m &= (c >= lo[ax]) & (c <= hi[ax])
     mesh.RegionTags[m] = tag
+
+
+@dataclass
+class ECore:
+    """Synthetic E-core problem: region tags 1 = air, 2 = iron (two legs and
+    a yoke), 3 = winding around the first leg; ``js_dir`` (Ne, 3) is the unit
+    current direction of the winding elements (zero elsewhere)."""
+    mesh: KuhnMesh
+    js_dir: np.ndarray
+    lengths: tuple
+
+
+def ecore_box(nx: int = 64, ny: int = 64, nz: int = 40, lengths=(0.16, 0.16, 0.10),
+              seed: int = 5) -> ECore:
+    """Two vertical iron legs joined by a yoke inside an air box, and a
+    rectangular winding around the first leg carrying current that circulates
+    counter-clockwise about +z.  Regions are fractions of the box, so any
+    resolution gives the same geometry; 64 x 64 x 40 cubes is 983,040 tets."""
+    m = kuhn_box(nx, ny, nz, lengths=lengths, seed=seed)
+    L = np.asarray(lengths, dtype=np.float64)
+    c = np.stack([centroid_axis(m, ax) for ax in range(3)], axis=1) / L
+
+    def box(lo, hi):
+        return np.all((c >= np.asarray(lo)) & (c <= np.asarray(hi)), axis=1)
+
+    leg1 = box((0.25, 0.375, 0.2), (0.375, 0.625, 0.85))
+    leg2 = box((0.625, 0.375, 0.2), (0.75, 0.625, 0.85))
+    yoke = box((0.25, 0.375, 0.7), (0.75, 0.625, 0.85))
+    iron = leg1 | leg2 | yoke
+    coil = box((0.15, 0.275, 0.3), (0.475, 0.725, 0.6)) & ~iron
+    m.RegionTags[:] = 1
+    m.RegionTags[iron] = 2
+    m.RegionTags[coil] = 3
+    # tangential direction around the first leg's axis (rectangular loop)
+    dx = (c[:, 0] - 0.3125) / 0.0625
+    dy = (c[:, 1] - 0.5) / 0.125
+    xside = np.abs(dx) >= np.abs(dy)
+    js = np.zeros((m.num_elems, 3))
+    js[:, 0] = np.where(xside, 0.0, -np.sign(dy))
+    js[:, 1] = np.where(xside, np.sign(dx), 0.0)
+    js[~coil] = 0.0
+    return ECore(m, js, tuple(float(v) for v in lengths))

@@ -1,0 +1,319 @@
+"""Backward-Euler nonlinear transient driver on the GPU engine.
+
+Mirror of TransientSolver.solve (src/solver/TransientSolver.m:38-267,
+274-311), pure-A formulation (no scalar potential space): per time step
+with source F(t),
+
+    R(x) = R_nu(x) + M_sigma (x - x_prev) / dt - F + reg * M_reg * x
+    J(x) = J_nu(x) + M_sigma / dt + reg * M_reg
+
+with the regularisation scale reg = RegScaleRel * mean|diag(J_nu + M_sigma/dt)|
+(positive entries) fixed at the first Newton iteration of the first step
+(:150-158), Dirichlet DOFs removed as BoundaryCondition.applyDirichlet does
+(:170), convergence when |R| < Tolerance or |R| / |R_first| < RelTolerance
+(:173-195), and the reference's backtracking line search (:207-237).
+
+Every matrix lives on the plan's structural CSC pattern on the device:
+J_nu and R_nu come from mpk_csc_jacobian, M_sigma and M_reg from
+mpk_csc_mass (once), the source from mpk_csc_source; the linear solve is the
+engine's Jacobi-PCG on the Dirichlet-applied operator (the reference's MUMPS
+direct solver is not available), with an inexact-Newton forcing term.
+
+A ``warm_start`` hook may supply the first Newton iterate of a step (the
+learned reluctivity predictor does); the plain iterate x_prev stays the
+fallback and the residual decides.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .assembler import Assembler, expand_material_map
+
+
+@dataclass
+class StepInfo:
+    step: int
+    time: float
+    newton_iters: int
+    pcg_iters: int
+    line_search_evals: int
+    residual: float
+    rel_residual: float
+    converged: bool
+    wall_ms: float
+    warm_start: str = "x_prev"
+    history: list = field(default_factory=list)
+
+
+class TransientSolver:
+    """TransientSolver with the reference's properties and defaults
+    (src/solver/TransientSolver.m:10-27)."""
+
+    def __init__(self, assembler: Assembler):
+        self.Assembler = assembler
+        self.Dt = 0.01
+        self.MaxIterations = 50
+        self.Tolerance = 1e-5
+        self.RelTolerance = 1e-6
+        self.UseLineSearch = True
+        self.MaxLineSearchIters = 10
+        self.Relaxation = 1.0
+        self.AutoRegularization = True
+        self.RegScaleRel = 1e-5
+        # linear solver (engine): Jacobi-PCG with forcing term
+        # eta_k = max(min(eta_max, gamma (|R_k| / |R_k-1|)^2), eta_floor * target / |R_k|)
+        self.LinearEtaMax = 1e-2
+        self.LinearGamma = 0.9
+        self.LinearEtaFloor = 0.5
+        self.LinearMaxIter = 20000
+        self.LinearCheckEvery = 10
+        self.Precision = "fp64"
+        # first Newton iterate of a step: "previous" (x_prev, the reference's
+        # choice, :159) or "extrapolated" (2 x_prev - x_prev2)
+        self.InitialGuess = "previous"
+        self.Verbose = False
+
+    # ------------------------------------------------------------------ setup
+    def _setup(self, space, matLib, sigmaMap, fixedDofs):
+        asm = self.Assembler
+        dev = torch.device("cuda", torch.cuda.current_device())
+        plan = asm.plan(space)
+        self.plan = plan
+        self.n = n = plan.n
+        self.dev = dev
+        self.sp = L.stream_ptr()
+        mesh = asm.Mesh
+        L.check(L.lib.mpk_plan_set_coords(plan.handle, L.ptr(mesh.P_dev), self.sp))
+        self.prec = L.PRECISIONS[self.Precision]
+        self.mat, self._mat_keep = asm.material_tables(matLib)
+        ntag = int(self.mat.num_tags)
+        if int(mesh.RegionTags.min()) < 1 or int(mesh.RegionTags.max()) > ntag:
+            raise L.MpkError(L.MPK_ERR_INDEX, "region tags outside material table")
+        self.tags = mesh.Tags_dev
+        self.qk = asm.stiffness_quadrature()
+        sigma = expand_material_map(sigmaMap, mesh.RegionTags)
+        self.sigma = torch.from_numpy(np.ascontiguousarray(sigma)).to(dev)
+        self.m_sigma = torch.empty(max(plan.nnz, 1), dtype=torch.float64, device=dev)
+        qm = asm.mass_quadrature(weighted=True)
+        L.check(L.lib.mpk_csc_mass(plan.handle, None, L.ptr(self.sigma), C.byref(qm), self.prec,
+                                   L.ptr(self.m_sigma), self.sp))
+        self.m_reg = None
+        if self.AutoRegularization:
+            self.m_reg = torch.empty_like(self.m_sigma)
+            qr = asm.mass_quadrature(weighted=False)
+            L.check(L.lib.mpk_csc_mass(plan.handle, None, None, C.byref(qr), self.prec,
+                                       L.ptr(self.m_reg), self.sp))
+        colptr, rowind = plan.pattern()
+        self.colptr = colptr.to(torch.int32).contiguous()
+        self.rowind = rowind.contiguous()
+        cols = torch.repeat_interleave(torch.arange(n, device=dev),
+                                       (colptr[1:] - colptr[:-1]).to(torch.int64))
+        self.diag_pos = torch.nonzero(self.rowind.to(torch.int64) == cols).flatten()
+        fixed = np.zeros(n, dtype=np.uint8)
+        if fixedDofs is not None:
+            fx = np.asarray(fixedDofs)
+            if fx.dtype == bool:
+                fixed[: min(len(fx), n)] = fx[:n]
+            else:
+                idx = fx.astype(np.int64).ravel()
+                idx = idx[(idx >= 1) & (idx <= n)]
+                fixed[idx - 1] = 1
+        self.fixed = torch.from_numpy(fixed).to(dev)
+        self.free = (self.fixed == 0).to(torch.float64)
+        ws = C.c_int64()
+        L.check(L.lib.mpk_pcg_work_size(n, C.byref(ws)))
+        self.work = torch.empty(ws.value, dtype=torch.float64, device=dev)
+        self.jv = torch.empty_like(self.m_sigma)
+        self.jsys = torch.empty_like(self.m_sigma)
+        self.cvals = torch.empty_like(self.m_sigma)
+        self.res = torch.empty(n, dtype=torch.float64, device=dev)
+        self.tmp = torch.empty(n, dtype=torch.float64, device=dev)
+        self.reg_abs = 0.0
+        self._c_key = None
+
+    # -------------------------------------------------------------- kernels
+    def _jacobian(self, x, calc_j: bool):
+        L.check(L.lib.mpk_csc_jacobian(self.plan.handle, None, L.ptr(self.tags), L.ptr(x),
+                                       C.byref(self.mat), C.byref(self.qk), int(calc_j),
+                                       self.prec, L.ptr(self.jv) if calc_j else None,
+                                       L.ptr(self.res), self.sp))
+
+    def _spmv(self, vals, x, y):
+        L.check(L.lib.mpk_spmv_sym(L.ptr(self.colptr), L.ptr(self.rowind), L.ptr(vals),
+                                   self.n, L.ptr(x), L.ptr(y), self.sp))
+
+    def _combine(self, out, c0, v0, c1=0.0, v1=None, c2=0.0, v2=None):
+        L.check(L.lib.mpk_csc_combine(self.plan.nnz, c0, L.ptr(v0), c1, L.ptr(v1), c2, L.ptr(v2),
+                                      0.0, None, L.ptr(out), self.sp))
+
+    def _constant_part(self, dt):
+        """C = M_sigma / dt + reg * M_reg (the x-linear part of R and J)."""
+        key = (dt, self.reg_abs)
+        if key != self._c_key:
+            if self.m_reg is not None and self.reg_abs > 0:
+                self._combine(self.cvals, 1.0 / dt, self.m_sigma, self.reg_abs, self.m_reg)
+            else:
+                self._combine(self.cvals, 1.0 / dt, self.m_sigma)
+            self._c_key = key
+
+    def residual(self, x, b, dt):
+        """Dirichlet-masked R(x) = R_nu(x) + C x - b and its norm."""
+        self._jacobian(x, calc_j=False)
+        self._constant_part(dt)
+        self._spmv(self.cvals, x, self.tmp)
+        r = (self.res + self.tmp - b) * self.free
+        return r, float(torch.linalg.vector_norm(r))
+
+    def _linear_solve(self, rhs, eta):
+        dx = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+        it = C.c_int()
+        rel = C.c_double()
+        L.check(L.lib.mpk_pcg_solve(L.ptr(self.colptr), L.ptr(self.rowind), L.ptr(self.jsys),
+                                    self.n, L.ptr(self.fixed), L.ptr(rhs), L.ptr(dx),
+                                    L.ptr(self.work), float(eta), self.LinearMaxIter,
+                                    self.LinearCheckEvery, C.byref(it), C.byref(rel), self.sp))
+        return dx, it.value
+
+    def solve_linear_system(self, vals, rhs, eta):
+        """PCG on a caller-assembled matrix of the plan's pattern (Dirichlet applied)."""
+        saved = self.jsys
+        self.jsys = vals
+        try:
+            return self._linear_solve(rhs * self.free, eta)
+        finally:
+            self.jsys = saved
+
+    # ---------------------------------------------------------------- solve
+    def solve(self, space_A, matLib, sigmaMap, sourceFunc, timeSteps, fixedDofs_A,
+              init_A=None, warm_start=None, on_step=None):
+        """Runs the time steps; returns (solutions [device tensors], infos).
+
+        sourceFunc(t) gives the source map (Assembler.expandSourceMap formats
+        or an (Ne, 3) device tensor).  warm_start(ctx) may return a candidate
+        first iterate for the step; on_step(ctx) is called after each step.
+        """
+        self._setup(space_A, matLib, sigmaMap, fixedDofs_A)
+        n = self.n
+        x_prev = torch.zeros(n, dtype=torch.float64, device=self.dev)
+        if init_A is not None:
+            a0 = torch.as_tensor(init_A, dtype=torch.float64, device=self.dev).flatten()
+            x_prev[: min(n, a0.numel())] = a0[:n]
+        x_prev2 = x_prev.clone()
+        results, infos = [], []
+        t_now = 0.0
+        for step, dt in enumerate(np.asarray(timeSteps, dtype=np.float64).ravel(), start=1):
+            torch.cuda.synchronize()
+            w0 = time.perf_counter()
+            t_now += float(dt)
+            F = self.Assembler.assembleSource(space_A, sourceFunc(t_now))
+            # b = F + M_sigma x_prev / dt
+            self._spmv(self.m_sigma, x_prev, self.tmp)
+            b = F + self.tmp / float(dt)
+            ctx = dict(step=step, t=t_now, dt=float(dt), x_prev=x_prev, x_prev2=x_prev2, b=b,
+                       solver=self)
+            x = x_prev.clone()
+            label = "x_prev"
+            rel_ref = None
+            extrap = self.InitialGuess == "extrapolated" and step > 2
+            if extrap and warm_start is None:
+                x, label = 2.0 * x_prev - x_prev2, "extrapolated"
+            if warm_start is not None and self.reg_abs > 0:
+                # the residual judges: warm start, extrapolation or x_prev
+                _, r_prev = self.residual(x_prev, b, dt)
+                ctx["r_prev"] = r_prev
+                rel_ref = r_prev  # the plain method's first residual
+                cand = warm_start(ctx)
+                if cand is not None:
+                    best, best_r = x, r_prev
+                    if extrap:
+                        xe = 2.0 * x_prev - x_prev2
+                        _, r_e = self.residual(xe, b, dt)
+                        if r_e < best_r:
+                            best, best_r, label = xe, r_e, "extrapolated"
+                    _, r_cand = self.residual(cand, b, dt)
+                    if r_cand < best_r:
+                        best, best_r, label = cand, r_cand, "warm"
+                    x = best
+            info = self._newton(step, t_now, float(dt), x, b, rel_ref)
+            info.warm_start = label
+            info.pcg_iters += int(ctx.get("warm_pcg", 0))
+            x = info.x
+            del info.x
+            torch.cuda.synchronize()
+            info.wall_ms = 1e3 * (time.perf_counter() - w0)
+            infos.append(info)
+            results.append(x.clone())
+            if on_step is not None:
+                ctx.update(x=x, info=info)
+                on_step(ctx)
+            x_prev2, x_prev = x_prev, x
+            if self.Verbose:
+                print(f"step {step} t={t_now:.4g}: {info.newton_iters} Newton, "
+                      f"{info.pcg_iters} PCG, res {info.residual:.3e} ({info.rel_residual:.2e}) "
+                      f"[{label}] {info.wall_ms:.1f} ms", flush=True)
+        return results, infos
+
+    def _newton(self, step, t, dt, x, b, rel_ref):
+        info = StepInfo(step, t, 0, 0, 0, math.inf, math.inf, False, 0.0)
+        initial_res = rel_ref
+        prev_norm = None
+        target = None
+        for it in range(1, self.MaxIterations + 1):
+            self._jacobian(x, calc_j=True)
+            if self.reg_abs == 0 and self.AutoRegularization:
+                self._combine(self.jsys, 1.0, self.jv, 1.0 / dt, self.m_sigma)
+                d = self.jsys[self.diag_pos].abs()
+                d = d[d > 0]
+                avg = float(d.mean()) if d.numel() else 1.0
+                if not math.isfinite(avg) or avg == 0:
+                    avg = 1.0
+                self.reg_abs = avg * self.RegScaleRel
+            self._constant_part(dt)
+            self._combine(self.jsys, 1.0, self.jv, 1.0, self.cvals)
+            self._spmv(self.cvals, x, self.tmp)
+            r = (self.res + self.tmp - b) * self.free
+            norm_res = float(torch.linalg.vector_norm(r))
+            info.history.append(norm_res)
+            if initial_res is None:
+                initial_res = norm_res if norm_res >= 1e-20 else 1.0
+            rel = norm_res / initial_res
+            info.newton_iters = it
+            info.residual, info.rel_residual = norm_res, rel
+            if norm_res < self.Tolerance or rel < self.RelTolerance:
+                info.converged = True
+                break
+            if norm_res > 1e20:
+                raise RuntimeError("Divergence detected (Res > 1e20). Check BCs or Time Step.")
+            target = max(self.Tolerance, self.RelTolerance * initial_res)
+            eta = self.LinearEtaMax
+            if prev_norm is not None:
+                eta = min(eta, self.LinearGamma * (norm_res / prev_norm) ** 2)
+            eta = max(eta, self.LinearEtaFloor * target / norm_res)
+            eta = min(eta, self.LinearEtaMax)
+            dx, nit = self._linear_solve(-r, eta)
+            info.pcg_iters += nit
+            if not bool(torch.isfinite(dx).all()):
+                break
+            alpha = self.Relaxation
+            if self.UseLineSearch:
+                for k in range(self.MaxLineSearchIters + 1):
+                    a_try = alpha * (0.5 ** k)
+                    x_try = x + a_try * dx
+                    _, r_try = self.residual(x_try, b, dt)
+                    info.line_search_evals += 1
+                    if r_try < norm_res or (it == 1 and k < 3):
+                        alpha = a_try
+                        break
+                    if r_try > norm_res * 10:
+                        alpha = alpha * 0.1
+            x = x + alpha * dx
+            prev_norm = norm_res
+        info.x = x
+        return info

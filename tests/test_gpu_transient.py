@@ -1,0 +1,53 @@
+"""GPU transient Newton driver against its CPU restatement (oracle kernels +
+scipy direct solves): same converged fields within the solver tolerance."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from magpackage_amd.boundary import outer_boundary_edge_mask
+from magpackage_amd.material import create_linear, marrocco
+from magpackage_amd.meshgen import ecore_box
+from tests.helpers import stiffness_tables
+from tests.transient_ref import transient_cpu
+
+pytestmark = pytest.mark.gpu
+
+
+def test_transient_matches_cpu_restatement(gpu):
+    import torch
+    from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
+    from magpackage_amd.boundary import find_outer_boundary_dofs
+    from magpackage_amd.transient import TransientSolver
+    ec = ecore_box(10, 10, 8, seed=3)
+    m = ec.mesh
+    lib = {1: create_linear(1.0), 2: marrocco(), 3: create_linear(1.0)}
+    sigma = np.where(m.RegionTags == 2, 2e6, 0.0)
+    amp = 3e7
+    steps = [1e-3] * 3
+    src = lambda t: ec.js_dir * amp * np.sin(2 * np.pi * 50 * t)
+    edges, _, _ = O.generate_edges(m.T)
+    fixed_cpu = outer_boundary_edge_mask(m.T, edges)
+    ref, ref_iters = transient_cpu(m, lib, sigma, src, steps, fixed_cpu)
+
+    M = Mesh.from_kuhn(m)
+    dh = DofHandler(M)
+    space = FunctionSpace("Nedelec", 1)
+    dh.distributeDofs(space)
+    asm = Assembler(M, dh)
+    fixed = find_outer_boundary_dofs(M, dh, space)
+    assert np.array_equal(fixed, fixed_cpu)
+    solver = TransientSolver(asm)
+    js = torch.from_numpy(ec.js_dir).cuda()
+    sols, infos = solver.solve(space, lib, {2: 2e6}, lambda t: js * (amp * np.sin(2 * np.pi * 50 * t)),
+                               steps, fixed)
+    mi = O.mesh_inputs(m)
+    _, curl, _ = stiffness_tables()
+    for k, (x, xr, info) in enumerate(zip(sols, ref, infos)):
+        print(f"step {k + 1}: gpu {info.newton_iters} Newton / {info.pcg_iters} PCG, "
+              f"cpu {ref_iters[k]} Newton, rel res {info.rel_residual:.2e}", flush=True)
+        assert info.converged
+        Bg = asm.computeElementB(x, space).cpu().numpy()
+        Br = O.element_b(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], xr, curl)
+        err = np.abs(Bg - Br).max() / np.abs(Br).max()
+        assert err < 1e-4, (k, err)
+        assert np.all(x.cpu().numpy()[fixed] == 0.0)

@@ -140,10 +140,12 @@ def test_kuhn_box_properties():
 
 
 def test_library_exports_header_symbols():
-    """libmpk_assembly.so loads and exports every function include/mpk_assembly.h declares."""
-    hdr = (ROOT / "include" / "mpk_assembly.h").read_text()
-    names = set(re.findall(r"^\s*(?:int|const char\*)\s+(mpk_\w+)\s*\(", hdr, flags=re.M))
-    assert len(names) >= 18
+    """libmpk_assembly.so loads and exports every function include/*.h declares."""
+    names = set()
+    for h in sorted((ROOT / "include").glob("*.h")):
+        names |= set(re.findall(r"^\s*(?:int|const char\*)\s+(mpk_\w+)\s*\(", h.read_text(),
+                                flags=re.M))
+    assert len(names) >= 30
     lib = C.CDLL(str(ROOT / "magpackage_amd" / "libmpk_assembly.so"))
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
@@ -161,3 +163,22 @@ def test_mex_arity_error_without_gpu():
     assert "MagPackage:Args" in str(e.value) and "Expected 7 inputs." in str(e.value)
     with pytest.raises(MpkError):
         mex.assemble_jacobian_kernel_mex(*range(16))
+
+
+def test_outer_boundary_edges_match_oracle():
+    """Host boundary finder (product path) against the oracle restatement of
+    BoundaryCondition.findOuterBoundaryDofs, on a box and on a mesh with a
+    hole (a removed block of cubes leaves inner boundary faces)."""
+    from magpackage_amd.boundary import outer_boundary_edge_mask
+    from magpackage_amd.meshgen import kuhn_box
+    from oracle import oracle as O
+    m = kuhn_box(5, 4, 3, seed=41)
+    edges, _, _ = O.generate_edges(m.T)
+    assert np.array_equal(outer_boundary_edge_mask(m.T, edges), O.boundary_edges(m.T, edges))
+    c = m.P[m.T - 1].mean(axis=1)
+    keep = ~((np.abs(c[:, 0] - 0.5) < 0.2) & (np.abs(c[:, 1] - 0.5) < 0.25))
+    T2 = m.T[keep]
+    edges2, _, _ = O.generate_edges(T2)
+    got = outer_boundary_edge_mask(T2, edges2)
+    assert np.array_equal(got, O.boundary_edges(T2, edges2))
+    assert got.sum() > O.boundary_edges(m.T, edges).sum()
