@@ -289,6 +289,39 @@ int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* va
                   double rel_tol, int max_iter, int check_every, int* iters_out,
                   double* rel_res_out, void* stream);
 
+/* Auxiliary (gradient) space of an edge-element system, for mpk_pcg_aux_solve.
+ * Free (non-Dirichlet) nodes are numbered 0..num_nodes-1.  Device arrays. */
+typedef struct mpk_aux_space {
+    int64_t num_nodes;
+    const int32_t* edge_nodes; /* [2n] per DOF: free-node numbers (a, b) of its edge's lower
+                                  and higher end (-1: a Dirichlet node) */
+    const int32_t* node_ptr;   /* [num_nodes+1] */
+    const int32_t* node_refs;  /* per node its edges: dof << 1 | (node is the higher end) */
+    const int32_t* ln_ptr;     /* Ln = G^T A G over the free nodes, CSR (symmetric) */
+    const int32_t* ln_idx;
+    const double* ln_vals;
+    int32_t sweeps;            /* damped Jacobi sweeps on Ln per application (>= 1) */
+    double omega;              /* their damping, e.g. 2/3 */
+} mpk_aux_space;
+
+/* Doubles of device workspace mpk_pcg_aux_solve needs. */
+int mpk_pcg_aux_work_size(int64_t n, int64_t num_nodes, int64_t* size_out);
+
+/* mpk_pcg_solve with the preconditioner M^-1 = D^-1 + G p(Ln) G^T (Jacobi plus
+ * a gradient-space correction, the auxiliary-space idea of Hiptmair and Xu):
+ * the near-null space of a weakly regularised curl-curl operator (gradients
+ * in non-conducting regions) is what stalls plain Jacobi-PCG.  G is the
+ * edge-node incidence (edge (a < b): phi_b - phi_a) restricted to free nodes,
+ * Ln = G^T A G (A's curl part vanishes on gradients, so Ln is the nodal
+ * matrix of A's mass part and does not change with the Newton iterate).
+ * p(Ln) = aux->sweeps damped Jacobi sweeps from zero (a fixed SPD polynomial:
+ * CG stays valid).  Same stopping rule, determinism and synchronisation as
+ * mpk_pcg_solve. */
+int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double* vals,
+                      int64_t n, const uint8_t* fixed, const mpk_aux_space* aux, const double* b,
+                      double* x, double* work, double rel_tol, int max_iter, int check_every,
+                      int* iters_out, double* rel_res_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,12 @@ class Material(C.Structure):
                 ("num_tags", C.c_int64), ("num_breaks", C.c_int64), ("num_coefs", C.c_int64)]
 
 
+class AuxSpace(C.Structure):
+    _fields_ = [("num_nodes", C.c_int64), ("edge_nodes", C.c_void_p), ("node_ptr", C.c_void_p),
+                ("node_refs", C.c_void_p), ("ln_ptr", C.c_void_p), ("ln_idx", C.c_void_p),
+                ("ln_vals", C.c_void_p), ("sweeps", C.c_int32), ("omega", C.c_double)]
+
+
 _vp = C.c_void_p
 _i64 = C.c_int64
 _i32 = C.c_int
@@ -96,6 +102,9 @@ _SIGS = {
     "mpk_spmv_sym": ([_vp, _vp, _vp, _i64, _vp, _vp, _vp], _i32),
     "mpk_csc_combine": ([_i64, _d, _vp, _d, _vp, _d, _vp, _d, _vp, _vp, _vp], _i32),
     "mpk_pcg_work_size": ([_i64, _pi64], _i32),
+    "mpk_pcg_aux_work_size": ([_i64, _i64, _pi64], _i32),
+    "mpk_pcg_aux_solve": ([_vp, _vp, _vp, _i64, _vp, C.POINTER(AuxSpace), _vp, _vp, _vp, _d, _i32,
+                           _i32, C.POINTER(C.c_int), C.POINTER(C.c_double), _vp], _i32),
     "mpk_mlp_sync_bf16": ([_vp, _vp, _vp], _i32),
     "mpk_mlp_forward": ([_vp, _vp, _vp, _i64, _vp, _vp], _i32),
     "mpk_mlp_work_size": ([_i64, _pi64], _i32),

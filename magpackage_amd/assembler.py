@@ -213,8 +213,9 @@ class Plan:
 
     def __del__(self):
         p = getattr(self, "_p", None)
-        if p is not None and p.value:
-            L.lib.mpk_plan_destroy(p)
+        lib = getattr(L, "lib", None)  # None while the interpreter shuts down
+        if p is not None and p.value and lib is not None:
+            lib.mpk_plan_destroy(p)
             self._p = C.c_void_p(None)
 
 

@@ -239,6 +239,123 @@ k_axpby4(int64_t nnz, double c0, const double* __restrict__ v0, double c1,
     }
 }
 
+// ---- auxiliary-space (gradient) correction ---------------------------------
+// M^-1 r = D^-1 r + G p(Ln) G^T r, with G the edge-node incidence restricted to
+// the free nodes and Ln = G^T A G (for a curl-curl operator plus a mass term,
+// G^T A G = G^T (mass part) G: the Jacobian's curl part vanishes on
+// gradients).  p(Ln) = `sweeps` damped Jacobi sweeps from zero, a fixed SPD
+// polynomial, so the preconditioner stays symmetric and CG applies.
+
+__global__ void __launch_bounds__(kB)
+k_update_r(int64_t n, const double* __restrict__ scal, const double* __restrict__ p,
+           const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
+           double* __restrict__ partial_rr) {
+    __shared__ double red[kB / 64];
+    const double alpha = scal[1];
+    double rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        rr += ri * ri;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rr += __shfl_xor(rr, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rr;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int w = 0; w < kB / 64; ++w) a += red[w];
+        partial_rr[blockIdx.x] = a;
+    }
+}
+
+// g = G^T r: node v sums +-r over its edges (ref = dof << 1 | v is the higher end).
+__global__ void __launch_bounds__(kB)
+k_gt(int64_t nv, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nref,
+     const double* __restrict__ r, double* __restrict__ g) {
+    for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kB) {
+        double s = 0.0;
+        for (int k = nptr[v]; k < nptr[v + 1]; ++k) {
+            const int ref = nref[k];
+            const double rv = r[ref >> 1];
+            s += (ref & 1) ? rv : -rv;
+        }
+        g[v] = s;
+    }
+}
+
+__global__ void __launch_bounds__(kB)
+k_jacobi_first(int64_t nv, const double* __restrict__ dninv, const double* __restrict__ g,
+               double omega, double* __restrict__ phi) {
+    for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kB)
+        phi[v] = omega * dninv[v] * g[v];
+}
+
+// phi_out = phi_in + omega D^-1 (g - Ln phi_in), kRowLanes lanes per row.
+__global__ void __launch_bounds__(kB)
+k_jacobi_sweep(const int32_t* __restrict__ lp, const int32_t* __restrict__ li,
+               const double* __restrict__ lv, int64_t nv, const double* __restrict__ dninv,
+               const double* __restrict__ g, double omega, const double* __restrict__ phi_in,
+               double* __restrict__ phi_out) {
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / kRowLanes);
+    for (int64_t v = (int64_t)blockIdx.x * (kB / kRowLanes) + threadIdx.x / kRowLanes; v < nv;
+         v += rows_per_grid) {
+        double s = 0.0;
+        for (int k = lp[v] + lane; k < lp[v + 1]; k += kRowLanes) s += lv[k] * phi_in[li[k]];
+#pragma unroll
+        for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kRowLanes);
+        if (lane == 0) phi_out[v] = phi_in[v] + omega * dninv[v] * (g[v] - s);
+    }
+}
+
+// z = D^-1 r + G phi (0 on fixed rows) and the partials of r.z.
+__global__ void __launch_bounds__(kB)
+k_z(int64_t n, const int2* __restrict__ en, const uint8_t* __restrict__ fixed,
+    const double* __restrict__ dinv, const double* __restrict__ r, const double* __restrict__ phi,
+    double* __restrict__ z, double* __restrict__ partial_rz) {
+    __shared__ double red[kB / 64];
+    double rz = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+        double zi = 0.0;
+        if (!(fixed && fixed[i])) {
+            const int2 ab = en[i];
+            zi = dinv[i] * r[i];
+            if (ab.y >= 0) zi += phi[ab.y];
+            if (ab.x >= 0) zi -= phi[ab.x];
+        }
+        z[i] = zi;
+        rz += r[i] * zi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rz += __shfl_xor(rz, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rz;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int w = 0; w < kB / 64; ++w) a += red[w];
+        partial_rz[blockIdx.x] = a;
+    }
+}
+
+__global__ void __launch_bounds__(kB)
+k_ln_dinv(const int32_t* __restrict__ lp, const int32_t* __restrict__ li,
+          const double* __restrict__ lv, int64_t nv, double* __restrict__ dninv) {
+    for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kB) {
+        double d = 0.0;
+        for (int k = lp[v]; k < lp[v + 1]; ++k)
+            if (li[k] == v) d += lv[k];
+        dninv[v] = (d > 0.0) ? 1.0 / d : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_copy(int64_t n, const double* __restrict__ a,
+                                             double* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB)
+        b[i] = a[i];
+}
+
 int grid_parts(int64_t n, int per_block) {
     int64_t g = (n + per_block - 1) / per_block;
     if (g > kMaxPartials) g = kMaxPartials;
@@ -336,6 +453,113 @@ int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* va
     }
     *iters_out = it;
     *rel_res_out = rel;
+    return MPK_OK;
+}
+
+int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double* vals,
+                      int64_t n, const uint8_t* fixed, const mpk_aux_space* aux, const double* b,
+                      double* x, double* work, double rel_tol, int max_iter, int check_every,
+                      int* iters_out, double* rel_res_out, void* stream) {
+    clear_error();
+    MPK_REQUIRE(n >= 0 && max_iter >= 0 && iters_out && rel_res_out && aux, MPK_ERR_ARGS,
+                "bad argument");
+    *iters_out = 0;
+    *rel_res_out = 0.0;
+    if (n == 0) return MPK_OK;
+    MPK_REQUIRE(colptr && rowind && vals && b && x && work, MPK_ERR_ARGS, "NULL argument");
+    const int64_t nv = aux->num_nodes;
+    MPK_REQUIRE(nv >= 0 && aux->sweeps >= 1 && aux->edge_nodes &&
+                    (nv == 0 || (aux->node_ptr && aux->node_refs && aux->ln_ptr &&
+                                 aux->ln_idx && aux->ln_vals)),
+                MPK_ERR_ARGS, "incomplete auxiliary space");
+    if (check_every < 1) check_every = 1;
+    hipStream_t st = as_stream(stream);
+    // work: dinv, r, z, p, q (n each), dninv, g, phi0, phi1 (nv each), partials, scal
+    double* dinv = work;
+    double* r = dinv + n;
+    double* z = r + n;
+    double* p = z + n;
+    double* q = p + n;
+    double* dninv = q + n;
+    double* g = dninv + nv;
+    double* phi0 = g + nv;
+    double* phi1 = phi0 + nv;
+    double* pa = phi1 + nv;
+    double* pb = pa + kMaxPartials;
+    double* pc = pb + kMaxPartials;
+    double* scal = pc + kMaxPartials;
+    const int g_vec = grid_parts(n, kB);
+    const int g_row = grid_parts(n, kB / kRowLanes);
+    const int g_nv = grid_parts(nv, kB);
+    const int g_nrow = grid_parts(nv, kB / kRowLanes);
+    const int2* en = reinterpret_cast<const int2*>(aux->edge_nodes);
+    // z = M^-1 r; partial r.z into pb
+    auto precondition = [&]() {
+        if (nv > 0) {
+            hipLaunchKernelGGL(k_gt, dim3(g_nv), dim3(kB), 0, st, nv, aux->node_ptr,
+                               aux->node_refs, r, g);
+            hipLaunchKernelGGL(k_jacobi_first, dim3(g_nv), dim3(kB), 0, st, nv, dninv, g,
+                               aux->omega, phi0);
+            double* a = phi0;
+            double* c = phi1;
+            for (int k = 1; k < aux->sweeps; ++k) {
+                hipLaunchKernelGGL(k_jacobi_sweep, dim3(g_nrow), dim3(kB), 0, st, aux->ln_ptr,
+                                   aux->ln_idx, aux->ln_vals, nv, dninv, g, aux->omega, a, c);
+                double* t = a;
+                a = c;
+                c = t;
+            }
+            hipLaunchKernelGGL(k_z, dim3(g_vec), dim3(kB), 0, st, n, en, fixed, dinv, r, a, z,
+                               pb);
+        } else {
+            hipLaunchKernelGGL(k_z, dim3(g_vec), dim3(kB), 0, st, n, en, fixed, dinv, r, phi0, z,
+                               pb);
+        }
+    };
+    if (nv > 0)
+        hipLaunchKernelGGL(k_ln_dinv, dim3(g_nv), dim3(kB), 0, st, aux->ln_ptr, aux->ln_idx,
+                           aux->ln_vals, nv, dninv);
+    // setup: dinv, r = b, x = 0 (k_setup also forms a Jacobi z that is replaced)
+    hipLaunchKernelGGL(k_setup, dim3(g_vec), dim3(kB), 0, st, colptr, rowind, vals, fixed, n, b,
+                       dinv, r, z, p, x, pb, pc);
+    precondition();
+    hipLaunchKernelGGL(k_setup_scal, dim3(1), dim3(kB), 0, st, pb, pc, g_vec, scal);
+    hipLaunchKernelGGL(k_copy, dim3(g_vec), dim3(kB), 0, st, n, z, p);
+    MPK_HIP_TRY(hipGetLastError());
+    double h[4];
+    MPK_HIP_TRY(hipMemcpyAsync(h, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MPK_HIP_TRY(hipStreamSynchronize(st));
+    const double r0 = std::sqrt(h[3]);
+    if (r0 == 0.0) return MPK_OK;
+    int it = 0;
+    double rel = 1.0;
+    while (it < max_iter) {
+        const int batch = (max_iter - it) < check_every ? (max_iter - it) : check_every;
+        for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL(k_spmv_dot, dim3(g_row), dim3(kB), 0, st, colptr, rowind, vals,
+                               fixed, n, p, q, pa);
+            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, pa, g_row, scal);
+            hipLaunchKernelGGL(k_update_r, dim3(g_vec), dim3(kB), 0, st, n, scal, p, q, x, r, pc);
+            precondition();
+            hipLaunchKernelGGL(k_beta, dim3(1), dim3(kB), 0, st, pb, pc, g_vec, scal);
+            hipLaunchKernelGGL(k_direction, dim3(g_vec), dim3(kB), 0, st, n, scal, z, p);
+        }
+        MPK_HIP_TRY(hipGetLastError());
+        it += batch;
+        MPK_HIP_TRY(hipMemcpyAsync(h, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        MPK_HIP_TRY(hipStreamSynchronize(st));
+        rel = std::sqrt(h[3]) / r0;
+        if (!(rel > rel_tol)) break;  // also stops on NaN
+    }
+    *iters_out = it;
+    *rel_res_out = rel;
+    return MPK_OK;
+}
+
+int mpk_pcg_aux_work_size(int64_t n, int64_t num_nodes, int64_t* size_out) {
+    clear_error();
+    MPK_REQUIRE(n >= 0 && num_nodes >= 0 && size_out, MPK_ERR_ARGS, "bad argument");
+    *size_out = 5 * n + 4 * num_nodes + 3 * (int64_t)kMaxPartials + 8;
     return MPK_OK;
 }
 

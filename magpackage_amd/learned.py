@@ -5,9 +5,14 @@ element of a nonlinear (B-H) region, a small MLP (include/mpk_mlp.h: 16 ->
 64 -> 64 -> 64 -> 1, tanh, BF16 weights on the matrix cores, FP32
 accumulation) maps
 
-    (B_prev, B_extrap = 2 B_prev - B_prev2, |B_prev|^2, |B_extrap|^2, region)
+    (B_prev, B_extrap = 2 B_prev - B_prev2, B_prev2, |B_prev|^2, |B_extrap|^2,
+     drive(t_next), region)
 
-to the log-reluctivity the converged solution of the next step will have.
+to the log-reluctivity the converged solution of the next step will have,
+learnt as a correction to log nu(|B_extrap|^2) (the B-H curve at the
+extrapolated field);
+``drive`` is an optional pair of global features of the prescribed source at
+the new time (for a sinusoidal winding: sin and cos of its phase).
 The warm start assembles K(nu_hat) + M_sigma/dt + reg M_reg (the Newton
 system's secant form, whose solution is the converged field when nu_hat is
 exact), solves it once, and hands the result to plain Newton; the driver
@@ -51,7 +56,7 @@ def _glorot(seed: int) -> np.ndarray:
 class NuPredictor:
     """Per-element reluctivity predictor for the elements of nonlinear regions."""
 
-    def __init__(self, assembler, matLib: dict, seed: int = 0):
+    def __init__(self, assembler, matLib: dict, seed: int = 0, drive=None):
         self.asm = assembler
         mesh = assembler.Mesh
         self.dev = torch.device("cuda", torch.cuda.current_device())
@@ -59,9 +64,10 @@ class NuPredictor:
                         and v.Type == "Nonlinear")
         if not nonlin:
             raise ValueError("no nonlinear material: nothing to predict")
-        if len(nonlin) > N_IN - 8:
+        if len(nonlin) > N_IN - 13:
             raise ValueError("too many nonlinear regions for the feature vector")
         self.tags = nonlin
+        self.drive = drive
         tags = mesh.RegionTags
         sel = np.isin(tags, nonlin)
         self.idx = torch.from_numpy(np.nonzero(sel)[0].astype(np.int64)).to(self.dev)
@@ -91,31 +97,44 @@ class NuPredictor:
         self.train_info = {}
 
     # ------------------------------------------------------------- features
-    def features(self, B_prev: torch.Tensor, B_prev2: torch.Tensor) -> torch.Tensor:
+    def features(self, B_prev: torch.Tensor, B_prev2: torch.Tensor, t_next: float = 0.0):
         bp = B_prev.index_select(0, self.idx)
-        be = 2.0 * bp - B_prev2.index_select(0, self.idx)
+        bq = B_prev2.index_select(0, self.idx)
+        be = 2.0 * bp - bq
         X = torch.zeros((self.idx.numel(), N_IN), dtype=torch.float32, device=self.dev)
         X[:, 0:3] = (bp / B_SCALE).float()
         X[:, 3:6] = (be / B_SCALE).float()
-        X[:, 6] = ((bp * bp).sum(dim=1) / S_SCALE).float()
-        X[:, 7] = ((be * be).sum(dim=1) / S_SCALE).float()
-        X[:, 8:8 + self.onehot.shape[1]] = self.onehot
+        X[:, 6:9] = (bq / B_SCALE).float()
+        X[:, 9] = ((bp * bp).sum(dim=1) / S_SCALE).float()
+        X[:, 10] = ((be * be).sum(dim=1) / S_SCALE).float()
+        if self.drive is not None:
+            d0, d1 = self.drive(t_next)
+            X[:, 11] = float(d0)
+            X[:, 12] = float(d1)
+        X[:, 13:13 + self.onehot.shape[1]] = self.onehot
         return X
 
-    def log_nu(self, B: torch.Tensor) -> torch.Tensor:
-        """log nu(|B|^2) of the selected elements from the B-H tables."""
-        s = (B.index_select(0, self.idx) ** 2).sum(dim=1).contiguous()
+    def log_nu(self, B: torch.Tensor, selected: bool = False) -> torch.Tensor:
+        """log nu(|B|^2) of the nonlinear-region elements from the B-H tables
+        (B over all elements, or over those elements when ``selected``)."""
+        Bs = B if selected else B.index_select(0, self.idx)
+        s = (Bs ** 2).sum(dim=1).contiguous()
         nu = torch.empty_like(s)
         L.check(L.lib.mpk_eval_nu(L.ptr(s), L.ptr(self.tags_sel), s.numel(), C.byref(self.mat),
                                   L.ptr(nu), None, L.stream_ptr()))
         return torch.log(nu)
 
-    def add_sample(self, B_prev, B_prev2, B_conv):
-        self.X.append(self.features(B_prev, B_prev2))
-        self.T.append(self.log_nu(B_conv).float())
+    def _base(self, B_prev, B_prev2) -> torch.Tensor:
+        bp = B_prev.index_select(0, self.idx)
+        return self.log_nu(2.0 * bp - B_prev2.index_select(0, self.idx), selected=True)
+
+    def add_sample(self, B_prev, B_prev2, B_conv, t_next: float = 0.0):
+        self.X.append(self.features(B_prev, B_prev2, t_next))
+        self.T.append((self.log_nu(B_conv) - self._base(B_prev, B_prev2)).float())
 
     # ------------------------------------------------------------- training
-    def fit(self, iters: int = 1500, batch: int = 8192, lr: float = 2e-3, seed: int = 0):
+    def fit(self, iters: int = 20000, batch: int = 8192, lr: float = 5e-3, seed: int = 0,
+            lr_final: float = 0.05):
         X = torch.cat(self.X)
         T = torch.cat(self.T)
         self.t_mean = float(T.mean())
@@ -143,7 +162,7 @@ class NuPredictor:
             if first is None:
                 first = float(loss)
             self.adam_step += 1
-            step_lr = lr * (0.1 ** (it / max(iters, 1)))
+            step_lr = lr * (lr_final ** (it / max(iters, 1)))
             L.check(L.lib.mpk_mlp_adam(L.ptr(self.params), L.ptr(self.m), L.ptr(self.v), L.ptr(g),
                                        self.adam_step, step_lr, 0.9, 0.999, 1e-8, L.ptr(self.wb),
                                        L.stream_ptr()))
@@ -164,22 +183,25 @@ class NuPredictor:
                                       X.shape[0], L.ptr(y), L.stream_ptr()))
         return y
 
-    def predict_nu(self, B_prev, B_prev2) -> torch.Tensor:
+    def predict_nu(self, B_prev, B_prev2, t_next: float = 0.0) -> torch.Tensor:
         """nu_hat for every element: the prediction in nonlinear regions, nu0 elsewhere."""
-        y = self._forward(self.features(B_prev, B_prev2))
+        y = self._forward(self.features(B_prev, B_prev2, t_next))
         nu = self.nu_lin.clone()
-        nu[self.idx] = torch.exp(y.double() * self.t_std + self.t_mean)
+        nu[self.idx] = torch.exp(self._base(B_prev, B_prev2) + y.double() * self.t_std +
+                                 self.t_mean)
         return nu
 
 
-def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5):
+def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5,
+                       eta_warm: float = 1e-2):
     """warm_start hook for TransientSolver.solve (None until the network is
     trained).  With nu_hat predicted from B_prev and its extrapolation, it
     solves the secant system A x = b, A = K(nu_hat) + M_sigma/dt + reg M_reg,
     for the correction around the extrapolated state x_e = 2 x_prev - x_prev2:
-    A d = b - A x_e, x0 = x_e + d, to the step's Newton target
-    (eta_floor * max(Tolerance, RelTolerance |R(x_prev)|) absolute), so that
-    when nu_hat is exact x0 already meets the convergence test."""
+    A d = b - A x_e, x0 = x_e + d, to relative accuracy eta_warm (never
+    tighter than the step's Newton target, eta_floor * max(Tolerance,
+    RelTolerance |R(x_prev)|) absolute): Newton polishes what the prediction
+    and the loose solve leave."""
     asm = predictor.asm
 
     def hook(ctx):
@@ -189,7 +211,7 @@ def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5):
         x_prev, x_prev2, b, dt = ctx["x_prev"], ctx["x_prev2"], ctx["b"], ctx["dt"]
         bp = asm.computeElementB(x_prev, space)
         bp2 = asm.computeElementB(x_prev2, space)
-        nu = predictor.predict_nu(bp, bp2).contiguous()
+        nu = predictor.predict_nu(bp, bp2, ctx["t"]).contiguous()
         kv = torch.empty_like(s.jsys)
         L.check(L.lib.mpk_csc_curl_curl(s.plan.handle, None, L.ptr(nu), C.byref(s.qk),
                                         L.PREC_FP64, L.ptr(kv), s.sp))
@@ -202,7 +224,7 @@ def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5):
         target = max(s.Tolerance, s.RelTolerance * ctx["r_prev"])
         if rn <= eta_floor * target:
             return xe
-        eta = min(max(eta_floor * target / rn, 1e-12), 1e-1)
+        eta = min(max(eta_floor * target / rn, eta_warm), 1e-1)
         d, nit = s.solve_linear_system(kv, r, eta)
         ctx["warm_pcg"] = nit
         return xe + d

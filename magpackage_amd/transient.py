@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -35,6 +36,106 @@ import torch
 
 from . import _lib as L
 from .assembler import Assembler, expand_material_map
+
+
+def divergence_free_source(assembler: Assembler, space, F, fixed, tol: float = 1e-10):
+    """Removes the discrete-gradient part of an assembled source vector:
+    F_df = F - M G phi with (G^T M G) phi = G^T F over the free nodes, M the
+    unweighted edge mass and G the edge-node incidence (edge (a < b):
+    phi_b - phi_a).  Then G^T F_df = 0: the source is the load of a
+    divergence-free current, as a winding's current is.  A current density
+    sampled per element (a rectangular loop's direction field) is not
+    discretely divergence-free; the reference's direct solver absorbs the
+    gradient part through the regularisation, an iterative solver stalls on
+    it.  Set-up work on the host (scipy CG on the nodal system)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    mesh = assembler.Mesh
+    plan = assembler.plan(space)
+    m = assembler.assembleMass(space, drop_zeros=False)
+    M = m.to_scipy()
+    edges = mesh.Edges.cpu().numpy().astype(np.int64)
+    emap = assembler.DofHandler.EntityMaps[space.toString()].cpu().numpy()
+    n = plan.n
+    fixed = np.asarray(fixed, dtype=bool)
+    act = emap > 0
+    rows = emap[act] - 1
+    E = edges[act]
+    G = sp.csr_matrix((np.tile([-1.0, 1.0], len(rows)), (np.repeat(rows, 2), (E - 1).ravel())),
+                      shape=(n, mesh.NumNodes))
+    bnodes = np.unique((E[fixed[rows]] - 1).ravel())
+    keep = np.setdiff1d(np.arange(mesh.NumNodes), bnodes)
+    Gi = G[:, keep]
+    f = F.detach().cpu().numpy() if torch.is_tensor(F) else np.asarray(F, dtype=np.float64)
+    f = np.where(fixed, 0.0, f)
+    Ln = (Gi.T @ M @ Gi).tocsr()
+    rhs = Gi.T @ f
+    phi, info = spla.cg(Ln, rhs, rtol=tol, maxiter=20000, M=sp.diags(1.0 / Ln.diagonal()))
+    if info != 0:
+        raise RuntimeError(f"nodal projection did not converge (info {info})")
+    fdf = f - M @ (Gi @ phi)
+    fdf[fixed] = 0.0
+    return torch.from_numpy(fdf).to(torch.device("cuda", torch.cuda.current_device()))
+
+
+def build_aux_space(assembler: Assembler, space_key: str, fixed, c_vals, colptr, rowind,
+                    sweeps: int = 2, omega: float = 2.0 / 3.0):
+    """Gradient space for mpk_pcg_aux_solve: G over the free nodes (nodes not on
+    a fixed edge) and Ln = G^T C G, C the system's mass part on the plan's
+    pattern (c_vals; the curl part of the system vanishes on gradients).  Host
+    set-up with scipy.  Returns (mpk_aux_space struct, device arrays it points
+    to, which the caller keeps alive)."""
+    import scipy.sparse as sp
+    mesh = assembler.Mesh
+    edges = mesh.Edges.cpu().numpy().astype(np.int64)
+    emap = assembler.DofHandler.EntityMaps[space_key].cpu().numpy().astype(np.int64)
+    fixed = (fixed.cpu().numpy() if torch.is_tensor(fixed) else np.asarray(fixed)).astype(bool)
+    n = fixed.size
+    act = emap > 0
+    rows = emap[act] - 1
+    E = edges[act] - 1
+    dnodes = np.zeros(mesh.NumNodes, dtype=bool)
+    dnodes[E[fixed[rows]].ravel()] = True
+    free = np.nonzero(~dnodes)[0]
+    free_id = np.full(mesh.NumNodes, -1, dtype=np.int64)
+    free_id[free] = np.arange(free.size)
+    nv = free.size
+    en = np.full((n, 2), -1, dtype=np.int32)
+    en[rows, 0] = free_id[E[:, 0]]
+    en[rows, 1] = free_id[E[:, 1]]
+    ma, mb = en[rows, 0] >= 0, en[rows, 1] >= 0
+    G = sp.csr_matrix((np.concatenate([-np.ones(int(ma.sum())), np.ones(int(mb.sum()))]),
+                       (np.concatenate([rows[ma], rows[mb]]),
+                        np.concatenate([en[rows, 0][ma], en[rows, 1][mb]]))), shape=(n, nv))
+    Gc = G.tocsc()
+    Gc.sort_indices()
+    refs = (Gc.indices.astype(np.int64) << 1) | (Gc.data > 0)
+    Cm = sp.csc_matrix((c_vals.cpu().numpy(), rowind.cpu().numpy(), colptr.cpu().numpy()),
+                       shape=(n, n))
+    Ln = (G.T @ Cm @ G).tocsr()
+    Ln.sort_indices()
+    dev = c_vals.device
+    keep = dict(
+        en=torch.from_numpy(en.reshape(-1)).to(dev),
+        nptr=torch.from_numpy(Gc.indptr.astype(np.int32)).to(dev),
+        nref=torch.from_numpy(refs.astype(np.int32)).to(dev),
+        lp=torch.from_numpy(Ln.indptr.astype(np.int32)).to(dev),
+        li=torch.from_numpy(Ln.indices.astype(np.int32)).to(dev),
+        lv=torch.from_numpy(Ln.data.astype(np.float64)).to(dev))
+    aux = L.AuxSpace()
+    aux.num_nodes = nv
+    aux.edge_nodes = L.ptr(keep["en"])
+    aux.node_ptr = L.ptr(keep["nptr"])
+    aux.node_refs = L.ptr(keep["nref"])
+    aux.ln_ptr = L.ptr(keep["lp"])
+    aux.ln_idx = L.ptr(keep["li"])
+    aux.ln_vals = L.ptr(keep["lv"])
+    aux.sweeps = int(sweeps)
+    aux.omega = float(omega)
+    ws = C.c_int64()
+    L.check(L.lib.mpk_pcg_aux_work_size(n, nv, C.byref(ws)))
+    keep["work"] = torch.empty(ws.value, dtype=torch.float64, device=dev)
+    return aux, keep
 
 
 @dataclass
@@ -50,6 +151,7 @@ class StepInfo:
     wall_ms: float
     warm_start: str = "x_prev"
     history: list = field(default_factory=list)
+    pcg_per_solve: list = field(default_factory=list)
 
 
 class TransientSolver:
@@ -68,12 +170,18 @@ class TransientSolver:
         self.AutoRegularization = True
         self.RegScaleRel = 1e-5
         # linear solver (engine): Jacobi-PCG with forcing term
-        # eta_k = max(min(eta_max, gamma (|R_k| / |R_k-1|)^2), eta_floor * target / |R_k|)
+        # eta_k = min(max(min(eta_max, gamma (|R_k| / |R_k-1|)^2),
+        #                 eta_floor * target / |R_k|), 0.5)
         self.LinearEtaMax = 1e-2
         self.LinearGamma = 0.9
         self.LinearEtaFloor = 0.5
         self.LinearMaxIter = 20000
         self.LinearCheckEvery = 10
+        # "aux": Jacobi + gradient-space correction (mpk_pcg_aux_solve);
+        # "jacobi": plain Jacobi-PCG
+        self.LinearPreconditioner = "aux"
+        self.AuxSweeps = 2
+        self.AuxOmega = 2.0 / 3.0
         self.Precision = "fp64"
         # first Newton iterate of a step: "previous" (x_prev, the reference's
         # choice, :159) or "extrapolated" (2 x_prev - x_prev2)
@@ -83,6 +191,7 @@ class TransientSolver:
     # ------------------------------------------------------------------ setup
     def _setup(self, space, matLib, sigmaMap, fixedDofs):
         asm = self.Assembler
+        self._space_key = space.toString()
         dev = torch.device("cuda", torch.cuda.current_device())
         plan = asm.plan(space)
         self.plan = plan
@@ -137,8 +246,21 @@ class TransientSolver:
         self.tmp = torch.empty(n, dtype=torch.float64, device=dev)
         self.reg_abs = 0.0
         self._c_key = None
+        self._aux_key = None
+        self._aux = None
+        self.timers = {}
 
     # -------------------------------------------------------------- kernels
+    def _tic(self):
+        if self.Verbose > 1:
+            torch.cuda.synchronize()
+        return time.perf_counter()
+
+    def _toc(self, name, t0):
+        if self.Verbose > 1:
+            torch.cuda.synchronize()
+            self.timers[name] = self.timers.get(name, 0.0) + 1e3 * (time.perf_counter() - t0)
+
     def _jacobian(self, x, calc_j: bool):
         L.check(L.lib.mpk_csc_jacobian(self.plan.handle, None, L.ptr(self.tags), L.ptr(x),
                                        C.byref(self.mat), C.byref(self.qk), int(calc_j),
@@ -165,16 +287,42 @@ class TransientSolver:
 
     def residual(self, x, b, dt):
         """Dirichlet-masked R(x) = R_nu(x) + C x - b and its norm."""
+        t0 = self._tic()
         self._jacobian(x, calc_j=False)
         self._constant_part(dt)
         self._spmv(self.cvals, x, self.tmp)
         r = (self.res + self.tmp - b) * self.free
-        return r, float(torch.linalg.vector_norm(r))
+        nr = float(torch.linalg.vector_norm(r))
+        self._toc("residual", t0)
+        return r, nr
+
+    def _build_aux(self):
+        self._aux = build_aux_space(self.Assembler, self._space_key, self.fixed, self.cvals,
+                                    self.colptr, self.rowind, self.AuxSweeps, self.AuxOmega)
 
     def _linear_solve(self, rhs, eta):
+        t0 = self._tic()
+        try:
+            return self._linear_solve_impl(rhs, eta)
+        finally:
+            self._toc("pcg", t0)
+
+    def _linear_solve_impl(self, rhs, eta):
         dx = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
         it = C.c_int()
         rel = C.c_double()
+        if self.LinearPreconditioner == "aux":
+            if self._aux_key != self._c_key:
+                self._build_aux()
+                self._aux_key = self._c_key
+            aux, keep = self._aux
+            L.check(L.lib.mpk_pcg_aux_solve(L.ptr(self.colptr), L.ptr(self.rowind),
+                                            L.ptr(self.jsys), self.n, L.ptr(self.fixed),
+                                            C.byref(aux), L.ptr(rhs), L.ptr(dx),
+                                            L.ptr(keep["work"]), float(eta), self.LinearMaxIter,
+                                            self.LinearCheckEvery, C.byref(it), C.byref(rel),
+                                            self.sp))
+            return dx, it.value
         L.check(L.lib.mpk_pcg_solve(L.ptr(self.colptr), L.ptr(self.rowind), L.ptr(self.jsys),
                                     self.n, L.ptr(self.fixed), L.ptr(rhs), L.ptr(dx),
                                     L.ptr(self.work), float(eta), self.LinearMaxIter,
@@ -196,8 +344,10 @@ class TransientSolver:
         """Runs the time steps; returns (solutions [device tensors], infos).
 
         sourceFunc(t) gives the source map (Assembler.expandSourceMap formats
-        or an (Ne, 3) device tensor).  warm_start(ctx) may return a candidate
-        first iterate for the step; on_step(ctx) is called after each step.
+        or an (Ne, 3) device tensor), or an assembled source vector (a 1-D
+        device tensor of NumGlobalDofs entries).  warm_start(ctx) may return a
+        candidate first iterate for the step; on_step(ctx) is called after
+        each step.
         """
         self._setup(space_A, matLib, sigmaMap, fixedDofs_A)
         n = self.n
@@ -212,7 +362,11 @@ class TransientSolver:
             torch.cuda.synchronize()
             w0 = time.perf_counter()
             t_now += float(dt)
-            F = self.Assembler.assembleSource(space_A, sourceFunc(t_now))
+            src = sourceFunc(t_now)
+            if torch.is_tensor(src) and src.dim() == 1:
+                F = src.to(torch.float64)
+            else:
+                F = self.Assembler.assembleSource(space_A, src)
             # b = F + M_sigma x_prev / dt
             self._spmv(self.m_sigma, x_prev, self.tmp)
             b = F + self.tmp / float(dt)
@@ -229,7 +383,9 @@ class TransientSolver:
                 _, r_prev = self.residual(x_prev, b, dt)
                 ctx["r_prev"] = r_prev
                 rel_ref = r_prev  # the plain method's first residual
+                t0 = self._tic()
                 cand = warm_start(ctx)
+                self._toc("warm_start", t0)
                 if cand is not None:
                     best, best_r = x, r_prev
                     if extrap:
@@ -243,7 +399,9 @@ class TransientSolver:
                     x = best
             info = self._newton(step, t_now, float(dt), x, b, rel_ref)
             info.warm_start = label
-            info.pcg_iters += int(ctx.get("warm_pcg", 0))
+            if "warm_pcg" in ctx:
+                info.pcg_iters += int(ctx["warm_pcg"])
+                info.pcg_per_solve.insert(0, int(ctx["warm_pcg"]))
             x = info.x
             del info.x
             torch.cuda.synchronize()
@@ -257,7 +415,14 @@ class TransientSolver:
             if self.Verbose:
                 print(f"step {step} t={t_now:.4g}: {info.newton_iters} Newton, "
                       f"{info.pcg_iters} PCG, res {info.residual:.3e} ({info.rel_residual:.2e}) "
-                      f"[{label}] {info.wall_ms:.1f} ms", flush=True)
+                      f"[{label}] {info.wall_ms:.1f} ms", file=sys.stderr, flush=True)
+                if self.Verbose > 1:
+                    print("    ms: " + " ".join(f"{k} {v:.1f}" for k, v in self.timers.items()),
+                          file=sys.stderr, flush=True)
+                    self.timers = {}
+                    ref = info.history[0] if rel_ref is None else rel_ref
+                    print("    residuals/R0 " + " ".join(f"{h / ref:.1e}" for h in info.history) +
+                          f" | PCG per solve {info.pcg_per_solve}", file=sys.stderr, flush=True)
         return results, infos
 
     def _newton(self, step, t, dt, x, b, rel_ref):
@@ -266,6 +431,7 @@ class TransientSolver:
         prev_norm = None
         target = None
         for it in range(1, self.MaxIterations + 1):
+            t0 = self._tic()
             self._jacobian(x, calc_j=True)
             if self.reg_abs == 0 and self.AutoRegularization:
                 self._combine(self.jsys, 1.0, self.jv, 1.0 / dt, self.m_sigma)
@@ -280,6 +446,7 @@ class TransientSolver:
             self._spmv(self.cvals, x, self.tmp)
             r = (self.res + self.tmp - b) * self.free
             norm_res = float(torch.linalg.vector_norm(r))
+            self._toc("jacobian", t0)
             info.history.append(norm_res)
             if initial_res is None:
                 initial_res = norm_res if norm_res >= 1e-20 else 1.0
@@ -292,13 +459,15 @@ class TransientSolver:
             if norm_res > 1e20:
                 raise RuntimeError("Divergence detected (Res > 1e20). Check BCs or Time Step.")
             target = max(self.Tolerance, self.RelTolerance * initial_res)
+            # Eisenstat-Walker forcing, never tighter than what reaching the
+            # target from this residual needs
             eta = self.LinearEtaMax
             if prev_norm is not None:
                 eta = min(eta, self.LinearGamma * (norm_res / prev_norm) ** 2)
-            eta = max(eta, self.LinearEtaFloor * target / norm_res)
-            eta = min(eta, self.LinearEtaMax)
+            eta = min(max(eta, self.LinearEtaFloor * target / norm_res), 0.5)
             dx, nit = self._linear_solve(-r, eta)
             info.pcg_iters += nit
+            info.pcg_per_solve.append(nit)
             if not bool(torch.isfinite(dx).all()):
                 break
             alpha = self.Relaxation

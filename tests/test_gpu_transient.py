@@ -51,3 +51,31 @@ def test_transient_matches_cpu_restatement(gpu):
         err = np.abs(Bg - Br).max() / np.abs(Br).max()
         assert err < 1e-4, (k, err)
         assert np.all(x.cpu().numpy()[fixed] == 0.0)
+
+
+def test_divergence_free_source(gpu):
+    """G^T F_df = 0 over the free nodes and F - F_df lies in M G (range of gradients)."""
+    import scipy.sparse as sp
+    import torch
+    from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
+    from magpackage_amd.boundary import find_outer_boundary_dofs
+    from magpackage_amd.transient import divergence_free_source
+    ec = ecore_box(10, 10, 8, seed=3)
+    M = Mesh.from_kuhn(ec.mesh)
+    dh = DofHandler(M)
+    space = FunctionSpace("Nedelec", 1)
+    dh.distributeDofs(space)
+    asm = Assembler(M, dh)
+    fixed = find_outer_boundary_dofs(M, dh, space)
+    F = asm.assembleSource(space, torch.from_numpy(ec.js_dir * 3e6).cuda())
+    Fdf = divergence_free_source(asm, space, F, fixed).cpu().numpy()
+    edges = M.Edges.cpu().numpy().astype(np.int64)
+    E = edges.shape[0]
+    G = sp.csr_matrix((np.tile([-1.0, 1.0], E), (np.repeat(np.arange(E), 2), (edges - 1).ravel())),
+                      shape=(E, M.NumNodes))
+    bn = np.unique((edges[fixed] - 1).ravel())
+    keep = np.setdiff1d(np.arange(M.NumNodes), bn)
+    f = np.where(fixed, 0.0, F.cpu().numpy())
+    assert np.linalg.norm(G[:, keep].T @ f) > 0.1 * np.linalg.norm(f)  # the raw load is not
+    assert np.linalg.norm(G[:, keep].T @ Fdf) < 1e-8 * np.linalg.norm(f)
+    assert np.all(Fdf[fixed] == 0.0)
