@@ -331,6 +331,55 @@ def run_ecore(args, emit=True):
     return res
 
 
+def run_mlp(args):
+    """MFMA throughput of the reluctivity MLP kernels: inference on 1M rows and
+    one training step (loss + backward) on a batch, HIP events around each
+    launch on the launch stream."""
+    import torch
+    from magpackage_amd import _lib as L
+    from magpackage_amd.learned import NPARAM, _glorot
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    p = torch.from_numpy(_glorot(0)).to(dev)
+    wb = torch.empty(9216, dtype=torch.int16, device=dev)
+    L.check(L.lib.mpk_mlp_sync_bf16(L.ptr(p), L.ptr(wb), L.stream_ptr()))
+    stream = torch.cuda.current_stream()
+    sp = L.stream_ptr(stream)
+    out = {}
+    # MFMA flops per row: 2 (16*64 + 64*64 + 64*64); backward: dH (2 layers)
+    # and dW (3 layers) products, 2 (2*64*64 + 16*64 + 2*64*64)
+    f_fwd = 2 * (16 * 64 + 2 * 64 * 64)
+    f_bwd = 2 * (4 * 64 * 64 + 16 * 64)
+    for name, n in (("forward", 1 << 20), ("loss_grad", 8192), ("loss_grad_1M", 1 << 20)):
+        X = torch.randn(n, 16, device=dev, dtype=torch.float32)
+        t = torch.randn(n, device=dev, dtype=torch.float32)
+        y = torch.empty(n, device=dev, dtype=torch.float32)
+        g = torch.empty(NPARAM, device=dev, dtype=torch.float32)
+        loss = torch.empty(1, device=dev, dtype=torch.float32)
+        ws = C.c_int64()
+        L.check(L.lib.mpk_mlp_work_size(n, C.byref(ws)))
+        work = torch.empty(ws.value, device=dev, dtype=torch.float32)
+
+        def step(ev_a=None, ev_b=None):
+            if ev_a is not None:
+                ev_a.record(stream)
+            if name == "forward":
+                L.check(L.lib.mpk_mlp_forward(L.ptr(p), L.ptr(wb), L.ptr(X), n, L.ptr(y), sp))
+            else:
+                L.check(L.lib.mpk_mlp_loss_grad(L.ptr(p), L.ptr(wb), L.ptr(X), L.ptr(t), n,
+                                                L.ptr(g), L.ptr(loss), L.ptr(work), sp))
+            if ev_b is not None:
+                ev_b.record(stream)
+
+        _, ms = timed_steps(step, stream, args.steps, args.warmup)
+        flops = n * (f_fwd if name == "forward" else f_fwd + f_bwd)
+        tf = flops / (ms * 1e-3) / 1e12
+        out[name] = {"rows": n, "ms": ms, "mfma_tflops": tf, "frac_of_2500": tf / 2500.0}
+        log(f"[mlp] {name}: {n} rows, {ms:.4f} ms, {tf:.2f} TFLOP/s on MFMA "
+            f"({100 * tf / 2500:.2f}% of 2.5 PFLOP/s dense bf16)")
+    print(json.dumps({"mlp": out}), flush=True)
+
+
 def timed_steps(step, stream, steps, warmup):
     """Runs warmup + timed steps; returns (ms per step over the timed region,
     mean ms of the bracketed kernel launches)."""
@@ -416,7 +465,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="freq128", choices=["freq128", "newton4m", "ecore"],
+    ap.add_argument("--workload", default="freq128", choices=["freq128", "newton4m", "ecore", "mlp"],
                     help="freq128: K + 1j*w*M_sigma complex128 on the 128^3 box (default); "
                          "newton4m: fused K(nu(B)) + M_sigma + J + R on the 100x100x67 box")
     ap.add_argument("--n", type=int, default=128, help="cubes per axis of the Kuhn box")
@@ -453,6 +502,8 @@ def main():
         print(json.dumps({"error": "this engine is measured on one GPU only; "
                                    "run with --gpus 1 (multi-GPU sharding is a design note)"}))
         sys.exit(2)
+    if args.workload == "mlp":
+        return run_mlp(args)
     if args.workload == "ecore":
         if args.steps == 20:
             args.steps = 200
