@@ -289,6 +289,39 @@ int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* va
                   double rel_tol, int max_iter, int check_every, int* iters_out,
                   double* rel_res_out, void* stream);
 
+/* ========================================================================== */
+/* 6. Batched assembly: many meshes x many frequencies in one launch           */
+/* ========================================================================== */
+/* No single reference call: the reference assembles K and M per mesh (the MEX
+ * kernels) and forms K + 1j*w*M per frequency (src/solver/FrequencySolver.m:
+ * 127-131); its HBFEM kernel (src/mex/assemble_hbfem_kernel_mex.cpp:25-260)
+ * stacks harmonic blocks of one mesh.  Here every item is one mesh with its
+ * plan (ring order, coordinates loaded with mpk_plan_set_coords), nu and
+ * sigma [ne] and an output of nfreq complex128 value sets [nfreq][2*nnz]
+ * (interleaved re, im), value set f = K + 1j*omegas[f]*M_sigma. */
+typedef struct mpk_batch_item {
+    const mpk_plan* plan;
+    const double* nu;
+    const double* sigma;
+    double* vals_c128;
+} mpk_batch_item;
+typedef struct mpk_batch mpk_batch;
+
+/* Builds the device descriptors and the tile table (copies; synchronises).
+ * precision: MPK_PREC_FP64 or MPK_PREC_FP32_LOCAL.  nfreq <= 64. */
+int mpk_batch_create(const mpk_batch_item* items, int64_t count, const double* omegas,
+                     int32_t nfreq, const mpk_quadrature* quad_k, const mpk_quadrature* quad_m,
+                     int precision, void* stream, mpk_batch** batch_out);
+/* Schedule: ring gather batch (2..4) and blocks per CU of a persistent grid
+ * (0: one block per 64-column tile).  Results do not depend on it. */
+int mpk_batch_schedule(mpk_batch* batch, int32_t ring_batch, int32_t blocks_per_cu);
+int mpk_batch_info(const mpk_batch* batch, int64_t* ntiles, int64_t* nnz_total,
+                   int64_t* lds_bytes);
+/* One launch over every item and frequency; no allocation, copy or
+ * synchronisation (capturable in a HIP graph). */
+int mpk_batch_run(const mpk_batch* batch, void* stream);
+int mpk_batch_destroy(mpk_batch* batch);
+
 /* Auxiliary (gradient) space of an edge-element system, for mpk_pcg_aux_solve.
  * Free (non-Dirichlet) nodes are numbered 0..num_nodes-1.  Device arrays. */
 typedef struct mpk_aux_space {

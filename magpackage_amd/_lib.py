@@ -54,6 +54,11 @@ class Material(C.Structure):
                 ("num_tags", C.c_int64), ("num_breaks", C.c_int64), ("num_coefs", C.c_int64)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("nu", C.c_void_p), ("sigma", C.c_void_p),
+                ("vals_c128", C.c_void_p)]
+
+
 class AuxSpace(C.Structure):
     _fields_ = [("num_nodes", C.c_int64), ("edge_nodes", C.c_void_p), ("node_ptr", C.c_void_p),
                 ("node_refs", C.c_void_p), ("ln_ptr", C.c_void_p), ("ln_idx", C.c_void_p),
@@ -103,6 +108,12 @@ _SIGS = {
     "mpk_csc_combine": ([_i64, _d, _vp, _d, _vp, _d, _vp, _d, _vp, _vp, _vp], _i32),
     "mpk_pcg_work_size": ([_i64, _pi64], _i32),
     "mpk_pcg_aux_work_size": ([_i64, _i64, _pi64], _i32),
+    "mpk_batch_create": ([C.POINTER(BatchItem), _i64, _vp, _i32, _pq, _pq, _i32, _vp,
+                          C.POINTER(_vp)], _i32),
+    "mpk_batch_schedule": ([_vp, _i32, _i32], _i32),
+    "mpk_batch_info": ([_vp, _pi64, _pi64, _pi64], _i32),
+    "mpk_batch_run": ([_vp, _vp], _i32),
+    "mpk_batch_destroy": ([_vp], _i32),
     "mpk_pcg_aux_solve": ([_vp, _vp, _vp, _i64, _vp, C.POINTER(AuxSpace), _vp, _vp, _vp, _d, _i32,
                            _i32, C.POINTER(C.c_int), C.POINTER(C.c_double), _vp], _i32),
     "mpk_mlp_sync_bf16": ([_vp, _vp, _vp], _i32),

@@ -34,35 +34,6 @@ __global__ void k_pad_coords(const double* __restrict__ P, int64_t nn, double4* 
     P4[i] = make_double4(P[3 * i], P[3 * i + 1], P[3 * i + 2], 0.0);
 }
 
-int make_csc_quad(const mpk_quadrature* q, bool curl, CscQuad& a) {
-    MPK_REQUIRE(q != nullptr, MPK_ERR_ARGS, "quadrature table is NULL");
-    MPK_REQUIRE(q->nq >= 1 && q->nq <= 8, MPK_ERR_ARGS, "nq=%d outside [1, 8]", q->nq);
-    std::memset(&a, 0, sizeof(a));
-    a.nq = q->nq;
-    double ws = 0.0;
-    for (int i = 0; i < q->nq; ++i) {
-        a.qw[i] = q->qw[i];
-        ws += q->qw[i];
-    }
-    a.wsum = ws;
-    const double* src = curl ? q->ref_curl : q->ref_val;
-    for (int i = 0; i < 18 * q->nq; ++i) a.ref[i] = src[i];
-    // barycentric moments Q_xy = sum_q w_q l_x l_y, l_1 = 1 - xi - eta - zeta
-    double Q[4][4] = {};
-    for (int k = 0; k < q->nq; ++k) {
-        const double xi = q->q_pts[3 * k], eta = q->q_pts[3 * k + 1], ze = q->q_pts[3 * k + 2];
-        const double l[4] = {((1.0 - xi) - eta) - ze, xi, eta, ze};
-        for (int x = 0; x < 4; ++x)
-            for (int y = 0; y < 4; ++y) Q[x][y] += q->qw[k] * l[x] * l[y];
-    }
-    double qd = 0.0, qo = 0.0;
-    for (int x = 0; x < 4; ++x)
-        for (int y = 0; y < 4; ++y) (x == y ? qd : qo) += Q[x][y];
-    a.qd = qd / 4.0;
-    a.qo = qo / 12.0;
-    return MPK_OK;
-}
-
 // The closed forms assume the Nedelec P1 reference curls of
 // nedelec_tet_p1.m:48 at every point (constant curl).
 bool standard_curl_table(const mpk_quadrature* q) {

@@ -2,6 +2,7 @@
 // structs, small vector math and the XCD-aware tile mapping.
 #pragma once
 
+#include <cstring>
 #include <type_traits>
 
 #include <cmath>
@@ -60,6 +61,38 @@ struct Vec3f {
 __device__ __forceinline__ Vec3f tof(Vec3 v) { return Vec3f{(float)v.x, (float)v.y, (float)v.z}; }
 __device__ __forceinline__ float dotf(Vec3f a, Vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
+// Kernel-argument copy of a quadrature table plus the rule's weight sum and
+// the symmetric barycentric moments the closed-form mass uses:
+// Q_xy = sum_q w_q l_x l_y, qd = mean diagonal, qo = mean off-diagonal.
+inline int make_csc_quad(const mpk_quadrature* q, bool curl, CscQuad& a) {
+    MPK_REQUIRE(q != nullptr, MPK_ERR_ARGS, "quadrature table is NULL");
+    MPK_REQUIRE(q->nq >= 1 && q->nq <= 8, MPK_ERR_ARGS, "nq=%d outside [1, 8]", q->nq);
+    std::memset(&a, 0, sizeof(a));
+    a.nq = q->nq;
+    double ws = 0.0;
+    for (int i = 0; i < q->nq; ++i) {
+        a.qw[i] = q->qw[i];
+        ws += q->qw[i];
+    }
+    a.wsum = ws;
+    const double* src = curl ? q->ref_curl : q->ref_val;
+    for (int i = 0; i < 18 * q->nq; ++i) a.ref[i] = src[i];
+    // barycentric moments Q_xy = sum_q w_q l_x l_y, l_1 = 1 - xi - eta - zeta
+    double Q[4][4] = {};
+    for (int k = 0; k < q->nq; ++k) {
+        const double xi = q->q_pts[3 * k], eta = q->q_pts[3 * k + 1], ze = q->q_pts[3 * k + 2];
+        const double l[4] = {((1.0 - xi) - eta) - ze, xi, eta, ze};
+        for (int x = 0; x < 4; ++x)
+            for (int y = 0; y < 4; ++y) Q[x][y] += q->qw[k] * l[x] * l[y];
+    }
+    double qd = 0.0, qo = 0.0;
+    for (int x = 0; x < 4; ++x)
+        for (int y = 0; y < 4; ++y) (x == y ? qd : qo) += Q[x][y];
+    a.qd = qd / 4.0;
+    a.qo = qo / 12.0;
+    return MPK_OK;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
 // Remap so that each XCD sweeps one contiguous range of column tiles: the
 // elements and nodes shared by neighbouring tiles then stay in that XCD's L2.
@@ -92,10 +125,10 @@ struct TileRange {
     int base, size;
 };
 template <class Op, class Acc>
-__device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* acc,
-                                                unsigned char* smem) {
+__device__ __forceinline__ TileRange tile_begin_at(const TileCtx& tc, Op& op, Acc* acc,
+                                                   unsigned char* smem, int64_t tile_id) {
     TileRange t;
-    t.tile_id = xcd_tile(blockIdx.x, gridDim.x);
+    t.tile_id = tile_id;
     t.j0 = t.tile_id * kW;
     t.jend = t.j0 + kW < tc.n ? t.j0 + kW : tc.n;
     t.base = tc.colptr[t.j0];
@@ -105,6 +138,11 @@ __device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* 
     for (int i = threadIdx.x; i < t.size * Planes<Op>::value; i += kW) acc[i] = Op::zero();
     if constexpr (Planes<Op>::value > 0) __syncthreads();
     return t;
+}
+template <class Op, class Acc>
+__device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* acc,
+                                                unsigned char* smem) {
+    return tile_begin_at(tc, op, acc, smem, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Tile epilogue: the finished accumulator range leaves as contiguous stores.

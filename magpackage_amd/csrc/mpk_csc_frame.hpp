@@ -475,14 +475,13 @@ __global__ void __launch_bounds__(kW) k_csc_frame(TileCtx tc, Op op) {
 // incidence (always valid addresses, no branches in the load stream); the
 // arithmetic of slots past the column's count is skipped.
 template <class Op, int kB>
-__global__ void __launch_bounds__(kW) k_csc_ring_batch(TileCtx tc, const int2* __restrict__ ends,
-                                                       Op op) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restrict__ ends, Op& op,
+                                          unsigned char* smem, int64_t tile_id) {
     using Acc = typename Op::Acc;
     using Ids = typename Op::Ids;
     using Elem = typename Op::Elem;
     Acc* acc = reinterpret_cast<Acc*>(smem);
-    const TileRange tr = tile_begin(tc, op, acc, smem);
+    const TileRange tr = tile_begin_at(tc, op, acc, smem, tile_id);
     const int lane = threadIdx.x;
     const int64_t j = tr.j0 + lane;
     if (j < tr.jend) {
@@ -527,6 +526,44 @@ __global__ void __launch_bounds__(kW) k_csc_ring_batch(TileCtx tc, const int2* _
     }
     tile_end(op, tr, acc);
 }
+
+template <class Op, int kB>
+__global__ void __launch_bounds__(kW) k_csc_ring_batch(TileCtx tc, const int2* __restrict__ ends,
+                                                       Op op) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    ring_tile<Op, kB>(tc, ends, op, smem, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// K + j w_f M_sigma for several frequencies from one pass: the LDS tile holds
+// (K, M) as FrameKM's does; the store writes one complex128 value set per
+// frequency, laid out [f][nnz] (HBFEM-style harmonic blocks, frequency sweeps).
+constexpr int kMaxFreq = 64;
+template <int kPrec>
+struct FrameKMF {
+    using Acc = double2;
+    using Ids = NoIds;
+    using Elem = double2;
+    struct Col {};
+    FrameKM<kPrec> km;
+    const double* __restrict__ omegas;  // [nfreq]
+    int nfreq;
+    int64_t nnz;
+    int acc_bytes_max;
+    __device__ static Acc zero() { return make_double2(0.0, 0.0); }
+    __device__ void block_init(unsigned char*) const {}
+    __device__ Col col_begin() const { return Col{}; }
+    __device__ void col_end(int64_t, Col&) const {}
+    __device__ void store(int64_t p, Acc a) const {
+        for (int f = 0; f < nfreq; ++f) km.out[f * nnz + p] = make_double2(a.x, omegas[f] * a.y);
+    }
+    __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
+    __device__ Elem elem(uint32_t e, const Ids& d) const { return km.elem(e, typename FrameKM<kPrec>::Ids{}); }
+    __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
+                            Acc* acc, Col&) const {
+        typename FrameKM<kPrec>::Col c{};
+        km.compute(nodes, el, f, pos, acc, c);
+    }
+};
 
 }  // namespace csc
 }  // namespace mpk
