@@ -345,7 +345,12 @@ def test_csc_jacobian(cfg1, prec, driver):
         assert abs(Jg.to_scipy() - Jref).max() / abs(Jref).max() < TOL[prec]
         assert rel_err(Rg, Rref) < TOL[prec]
     _, R0 = asm.assembleJacobian(space, a, lib, calcJ=False, precision=prec)
-    assert np.array_equal(R0.cpu().numpy(), Rg)
+    if prec == "fp64_exact":
+        assert np.array_equal(R0.cpu().numpy(), Rg)
+    else:
+        # the residual-only pass is another kernel instantiation (no LDS tile):
+        # same arithmetic, FMA contraction may round differently
+        assert rel_err(R0.cpu().numpy(), Rg) < 1e-13
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
