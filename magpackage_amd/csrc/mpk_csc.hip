@@ -15,10 +15,11 @@
 //   FP64_EXACT  mpk_csc_exact.hpp: reference operation order, element order,
 //               bit-identical to MEX kernel + sparse() in triplet order.
 //   FP64        mpk_csc_frame.hpp: closed forms with FMA, <= 1e-12 relative.
-//   FP32_LOCAL  mpk_csc_frame.hpp: FP32 block entries, FP64 geometry and
-//               accumulation, <= 1e-6 relative.
+//   FP32_LOCAL  mpk_csc_frame.hpp: FP32 block entries and element state,
+//               FP64 geometry and accumulation, <= 1e-6 relative.
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "mpk_csc_exact.hpp"
 #include "mpk_csc_frame.hpp"
@@ -70,13 +71,68 @@ int load_coords(const mpk_plan* plan, const double* P, hipStream_t st);
 // (nedelec_tet_p1.m:48; the Piola map of assemble_jacobian_kernel_mex.cpp:
 // 94-113), then (nu, dnu) = evaluate_nu_derivative(|B|^2, tag)
 // (MexMaterialUtils.hpp:24-87) with the B-H tables staged in LDS.
+// The FP64 state (StateView) is 48 B per element; the FP32-local one
+// (StateView32) rounds the same values to FP32, 24 B per element.
 constexpr int kStateBlock = 256;
 
+// put_state stores the state and returns (dnu, sigma) as the assembly
+// kernels will read them back.
+__device__ inline double2 put_state(double4* s0, double2* s1, int64_t e, double bx, double by,
+                                    double bz, double nu, double dnu, double sg) {
+    s0[e] = make_double4(bx, by, bz, nu);
+    s1[e] = make_double2(dnu, sg);
+    return make_double2(dnu, sg);
+}
+__device__ inline double2 put_state(float4* s0, float2* s1, int64_t e, double bx, double by,
+                                    double bz, double nu, double dnu, double sg) {
+    s0[e] = make_float4((float)bx, (float)by, (float)bz, (float)nu);
+    const float2 v = make_float2((float)dnu, (float)sg);
+    s1[e] = v;
+    return make_double2(v.x, v.y);
+}
+
+// Clears the tile class flags and the class-1 tile count.
+__global__ void k_clear_tiles(int32_t* __restrict__ flag, int64_t ntiles,
+                              int32_t* __restrict__ count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ntiles) flag[i] = 0;
+    if (i == 0) *count = 0;
+}
+
+// Lists the class-1 tiles: ascending within each block of 256 tiles, the
+// blocks' ranges appended in arrival order (the tiles are independent, so the
+// list order changes no result).
+constexpr int kListBlock = 256;
+__global__ void __launch_bounds__(kListBlock)
+k_tile_list(const int32_t* __restrict__ flag, int64_t ntiles, int32_t* __restrict__ list,
+            int32_t* __restrict__ count) {
+    __shared__ int wsum[kListBlock / 64];
+    __shared__ int base;
+    const int64_t t = (int64_t)blockIdx.x * kListBlock + threadIdx.x;
+    const bool on = t < ntiles && flag[t] != 0;
+    const unsigned long long m = __ballot(on);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int i = 0; i < kListBlock / 64; ++i) tot += wsum[i];
+        base = tot ? atomicAdd(count, tot) : 0;
+    }
+    __syncthreads();
+    int off = base;
+    for (int i = 0; i < w; ++i) off += wsum[i];
+    if (on) list[off + before] = (int32_t)t;
+}
+
+template <class S0, class S1>
 __global__ void __launch_bounds__(kStateBlock)
 k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
              const double4* __restrict__ P4, int64_t ne, const double* __restrict__ sol,
              const int32_t* __restrict__ tags, const double* __restrict__ sigma, MaterialView mat,
-             int ntags, int nbrk, int ncoe, double4* __restrict__ s0, double2* __restrict__ s1) {
+             int ntags, int nbrk, int ncoe, S0* __restrict__ s0, S1* __restrict__ s1,
+             int32_t* __restrict__ tile_flag, int flag_sigma, int flag_dnu) {
     __shared__ double tab[kMatLds];
     const double* brk = mat.brk;
     const double* coe = mat.coe;
@@ -116,14 +172,31 @@ k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
     tag = tag < 1 ? 1 : (tag > ntags ? ntags : tag);
     double nu, dnu;
     eval_nu(bx * bx + by * by + bz * bz, tag, mat, brk, coe, nu, dnu);
-    s0[e] = make_double4(bx, by, bz, nu);
-    s1[e] = make_double2(dnu, sigma ? sigma[e] : 1.0);
+    const double2 ds = put_state(s0, s1, e, bx, by, bz, nu, dnu, sigma ? sigma[e] : 1.0);
+    // Tile classes for the split Newton set: an element with a D term (dnu != 0)
+    // or an M term (|sigma| > 1e-12), the tests FrameNewton applies, marks the
+    // column tiles of its 6 DOFs.  Every writer stores the same value, so plain
+    // stores need no atomics.
+    if (tile_flag && ((flag_dnu && ds.x != 0.0) || (flag_sigma && fabs(ds.y) > 1e-12))) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const int d = cell[6 * e + i];
+            if (d >= 0) tile_flag[d / kTileCols] = 1;
+        }
+    }
 }
 
 // Loads P (or checks that the plan holds coordinates), makes sure the state
 // buffer exists and fills it.  The buffer is allocated on the first call.
+// With tile_flag ([2 ntiles + 1] int32: class flags, class-1 tile list,
+// class-1 count), the tile classes and the list of class-1 tiles are written
+// too: flag_sigma / flag_dnu say which terms (M / D) classify a tile.
+template <class SV>
 int elem_state(const mpk_plan* plan, const double* P, const int32_t* tags, const double* sol,
-               const double* sigma, const mpk_material* mat, hipStream_t st, StateView* sv) {
+               const double* sigma, const mpk_material* mat, hipStream_t st, SV* sv,
+               int32_t* tile_flag = nullptr, bool flag_sigma = false, bool flag_dnu = false) {
+    using S0 = typename SV::S0;
+    using S1 = typename SV::S1;
     if (P) {
         if (int rc = load_coords(plan, P, st)) return rc;
     } else {
@@ -132,16 +205,28 @@ int elem_state(const mpk_plan* plan, const double* P, const int32_t* tags, const
     }
     if (plan->ne == 0) return MPK_OK;
     MPK_REQUIRE(plan->cell, MPK_ERR_ARGS, "plan without cell DOFs");
+    // sized for the FP64 state; the FP32 one uses the first half
     if (!plan->buf_state.p) MPK_HIP_TRY(plan->buf_state.alloc((size_t)plan->ne * 48));
-    double4* s0 = plan->buf_state.as<double4>();
-    double2* s1 = reinterpret_cast<double2*>(s0 + plan->ne);
+    S0* s0 = plan->buf_state.as<S0>();
+    S1* s1 = reinterpret_cast<S1*>(s0 + plan->ne);
+    const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
+    if (tile_flag)
+        hipLaunchKernelGGL(k_clear_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, st,
+                           tile_flag, ntiles, tile_flag + 2 * ntiles);
     MaterialView mv{mat->linear_nu, mat->is_nonlinear, mat->max_bsq,    mat->breaks,
                     mat->coefs,     mat->spline_start, mat->spline_count, mat->coef_start};
-    hipLaunchKernelGGL(k_elem_state, dim3(grid_for(plan->ne, kStateBlock)), dim3(kStateBlock), 0,
+    hipLaunchKernelGGL((k_elem_state<S0, S1>), dim3(grid_for(plan->ne, kStateBlock)),
+                       dim3(kStateBlock), 0,
                        st, plan->T, plan->cell, plan->P4, plan->ne, sol, tags, sigma, mv,
-                       (int)mat->num_tags, (int)mat->num_breaks, (int)mat->num_coefs, s0, s1);
+                       (int)mat->num_tags, (int)mat->num_breaks, (int)mat->num_coefs, s0, s1,
+                       tile_flag, (int)flag_sigma, (int)flag_dnu);
     MPK_HIP_TRY(hipGetLastError());
-    *sv = StateView{s0, s1};
+    if (tile_flag) {
+        hipLaunchKernelGGL(k_tile_list, dim3(grid_for(ntiles, kListBlock)), dim3(kListBlock), 0,
+                           st, tile_flag, ntiles, tile_flag + ntiles, tile_flag + 2 * ntiles);
+        MPK_HIP_TRY(hipGetLastError());
+    }
+    *sv = SV{s0, s1};
     return MPK_OK;
 }
 
@@ -168,15 +253,89 @@ int load_coords(const mpk_plan* plan, const double* P, hipStream_t st) {
     return MPK_OK;
 }
 
+// Incidences per gather round of the ring driver: the op's kRing (else
+// kRingBatch) unless the environment sets MPK_RING_BATCH to 2, 3, 4, 6 or 8 (a tuning knob: results
+// are identical for every value, the per-entry summation order is the same).
+template <class Op, class = void>
+struct RingDefault {
+    static constexpr int value = kRingBatch;
+};
+template <class Op>
+struct RingDefault<Op, std::void_t<decltype(Op::kRing)>> {
+    static constexpr int value = Op::kRing;
+};
+template <class Op>
+int ring_batch() {
+    const char* s = std::getenv("MPK_RING_BATCH");
+    if (!s) return RingDefault<Op>::value;
+    const int v = std::atoi(s);
+    return (v == 2 || v == 3 || v == 4 || v == 6 || v == 8) ? v : RingDefault<Op>::value;
+}
+
 // Closed-form kernels read the plan's 32-byte-padded coordinate copy.  With P
 // given it is refreshed from P first (stream-ordered, so a P changed between
 // calls is picked up); with P == NULL the copy made by the last
 // mpk_plan_set_coords (or the last call with P) is used.  The ring driver runs
 // when the plan has ring records, unless the environment sets
 // MPK_CSC_DRIVER=element (element-order driver, for tests and timing).
+bool ring_driver(const mpk_plan* plan) {
+    const char* drv = std::getenv("MPK_CSC_DRIVER");
+    return plan->ring && !(drv && std::strcmp(drv, "element") == 0);
+}
+
+// Persistent launch of the ring driver over the tiles of a device list
+// (list[0, *count)): as many blocks as fit on the device at the op's LDS size.
+template <class Op>
+int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t st,
+                     const int32_t* list, const int32_t* count) {
+    size_t lds = 0;
+    if (int rc = lds_bytes(plan, op, acc_size, 0, &lds)) return rc;
+    if (plan->num_cus == 0) {
+        int cus = 0;
+        MPK_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                          plan->device));
+        plan->num_cus = cus > 0 ? cus : 256;
+    }
+    constexpr size_t kLdsPerCu = 160 * 1024;
+    size_t per_cu = lds ? kLdsPerCu / lds : 16;
+    per_cu = per_cu < 1 ? 1 : (per_cu > 16 ? 16 : per_cu);
+    const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
+    // the kernel splits the list over the 8 XCDs by block index: the grid is a
+    // multiple of 8 so that every XCD's range has blocks
+    int64_t g = (int64_t)plan->num_cus * (int64_t)per_cu;
+    g = g < ntiles ? g : ntiles;
+    g = (g + 7) / 8 * 8;
+    const unsigned grid = (unsigned)(g > 8 ? g : 8);
+    const TileCtx tc = tile_ctx(plan);
+    switch (ring_batch<Op>()) {
+        case 2:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 2>), dim3(grid), dim3(kW), lds, st, tc,
+                               plan->col_ends, op, list, count);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 4>), dim3(grid), dim3(kW), lds, st, tc,
+                               plan->col_ends, op, list, count);
+            break;
+        case 6:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 6>), dim3(grid), dim3(kW), lds, st, tc,
+                               plan->col_ends, op, list, count);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 8>), dim3(grid), dim3(kW), lds, st, tc,
+                               plan->col_ends, op, list, count);
+            break;
+        default:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 3>), dim3(grid), dim3(kW), lds, st, tc,
+                               plan->col_ends, op, list, count);
+    }
+    MPK_HIP_TRY(hipGetLastError());
+    return MPK_OK;
+}
+
+// sel / want: ring driver only, run the tiles whose class flag equals want.
 template <class Op>
 int launch_frame(const mpk_plan* plan, const double* P, Op op, size_t acc_size, size_t extra,
-                 hipStream_t st) {
+                 hipStream_t st, const int32_t* sel = nullptr, int32_t want = 0) {
     if (plan->n == 0) return MPK_OK;
     size_t lds = 0;
     if (int rc = lds_bytes(plan, op, acc_size, extra, &lds)) return rc;
@@ -186,16 +345,91 @@ int launch_frame(const mpk_plan* plan, const double* P, Op op, size_t acc_size, 
         MPK_REQUIRE(plan->coords_loaded, MPK_ERR_ARGS,
                     "P is NULL and the plan holds no coordinates (mpk_plan_set_coords)");
     }
-    const char* drv = std::getenv("MPK_CSC_DRIVER");
-    const bool element = drv && std::strcmp(drv, "element") == 0;
     const unsigned grid = grid_for(plan->n, kW);
-    if (plan->ring && !element)
-        hipLaunchKernelGGL((k_csc_ring_batch<Op, kRingBatch>), dim3(grid), dim3(kW), lds, st,
-                           tile_ctx(plan), plan->col_ends, op);
-    else
+    if (ring_driver(plan)) {
+        const TileCtx tc = tile_ctx(plan);
+        switch (ring_batch<Op>()) {
+            case 2:
+                hipLaunchKernelGGL((k_csc_ring_batch<Op, 2>), dim3(grid), dim3(kW), lds, st, tc,
+                                   plan->col_ends, op, sel, want);
+                break;
+            case 4:
+                hipLaunchKernelGGL((k_csc_ring_batch<Op, 4>), dim3(grid), dim3(kW), lds, st, tc,
+                                   plan->col_ends, op, sel, want);
+                break;
+            case 6:
+                hipLaunchKernelGGL((k_csc_ring_batch<Op, 6>), dim3(grid), dim3(kW), lds, st, tc,
+                                   plan->col_ends, op, sel, want);
+                break;
+            case 8:
+                hipLaunchKernelGGL((k_csc_ring_batch<Op, 8>), dim3(grid), dim3(kW), lds, st, tc,
+                                   plan->col_ends, op, sel, want);
+                break;
+            default:
+                hipLaunchKernelGGL((k_csc_ring_batch<Op, 3>), dim3(grid), dim3(kW), lds, st, tc,
+                                   plan->col_ends, op, sel, want);
+        }
+    } else {
+        MPK_REQUIRE(!sel, MPK_ERR_ARGS, "tile selection needs the ring driver");
         hipLaunchKernelGGL(k_csc_frame<Op>, dim3(grid), dim3(kW), lds, st, tile_ctx(plan), op);
+    }
     MPK_HIP_TRY(hipGetLastError());
     return MPK_OK;
+}
+
+// The fused Newton set.  Ring driver: the element-state kernel also classes
+// the column tiles; tiles that no M or D term reaches run FrameNewtonLin (one
+// LDS plane), the others FrameNewton (two or three planes), in two launches.
+// Each tile is written by exactly one of them.
+template <int kPrec, class SV>
+int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const double* sol_a,
+               const mpk_material* mat, const double* sigma, const CscQuad& qk,
+               const CscQuad& qm, double* k_vals, double* m_vals, double* j_vals,
+               double* residual, hipStream_t st) {
+    const bool with_m = m_vals != nullptr;
+    const bool split = plan->n > 0 && ring_driver(plan);
+    int32_t* flags = nullptr;
+    const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
+    if (split) {
+        if (!plan->buf_tile_flag.p)
+            MPK_HIP_TRY(plan->buf_tile_flag.alloc((size_t)(2 * ntiles + 1) * sizeof(int32_t)));
+        flags = plan->buf_tile_flag.as<int32_t>();
+    }
+    // The classes depend on sigma and dnu, not on which outputs are requested
+    // (a call for a subset of the outputs runs each tile with the same op as
+    // the full call).  Without sigma the state holds sigma = 1: M then reaches
+    // every tile.
+    SV sv{};
+    if (int rc = elem_state(plan, P, tags, sol_a, sigma, mat, st, &sv, flags,
+                            with_m || sigma != nullptr, true))
+        return rc;
+    if (split) {
+        if (int rc = launch_frame(plan, nullptr,
+                                  FrameNewtonLin<kPrec, SV>{sv, k_vals, m_vals, j_vals, residual,
+                                                            qk, 0},
+                                  sizeof(double), 0, st, flags, 0))
+            return rc;
+        const int32_t* list = flags + ntiles;
+        const int32_t* count = flags + 2 * ntiles;
+        if (with_m)
+            return launch_ring_list(plan,
+                                    FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals, j_vals,
+                                                                 residual, qk, qm, 0, 0},
+                                    3 * sizeof(double), st, list, count);
+        return launch_ring_list(plan,
+                                FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr, j_vals,
+                                                              residual, qk, qm, 0, 0},
+                                2 * sizeof(double), st, list, count);
+    }
+    if (with_m)
+        return launch_frame(plan, nullptr,
+                            FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals, j_vals, residual, qk,
+                                                         qm, 0, 0},
+                            3 * sizeof(double), 0, st, flags, 1);
+    return launch_frame(plan, nullptr,
+                        FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr, j_vals, residual, qk,
+                                                      qm, 0, 0},
+                        2 * sizeof(double), 0, st, flags, 1);
 }
 
 }  // namespace
@@ -315,26 +549,24 @@ int mpk_csc_jacobian(const mpk_plan* plan, const double* P, const int32_t* tags,
             return launch_exact(plan, P, ExactJac<false>{P, tags, sol_a, mv, ntags, nbrk, ncoe,
                                                       j_vals, residual, qa, 0, nullptr, nullptr},
                                 8, extra, st);
-        case MPK_PREC_FP64:
-        case MPK_PREC_FP32_LOCAL: {
+        case MPK_PREC_FP64: {
             StateView sv{};
             if (int rc = elem_state(plan, P, tags, sol_a, nullptr, mat, st, &sv)) return rc;
-            if (precision == MPK_PREC_FP64) {
-                if (calc_j)
-                    return launch_frame(plan, nullptr,
-                                        FrameJac<MPK_PREC_FP64, true>{sv, j_vals, residual, qa, 0},
-                                        8, 0, st);
-                return launch_frame(plan, nullptr,
-                                    FrameJac<MPK_PREC_FP64, false>{sv, j_vals, residual, qa, 0}, 0,
-                                    0, st);
-            }
             if (calc_j)
-                return launch_frame(
-                    plan, nullptr,
-                    FrameJac<MPK_PREC_FP32_LOCAL, true>{sv, j_vals, residual, qa, 0}, 8, 0, st);
+                return launch_frame(plan, nullptr,
+                                    FrameJac<MPK_PREC_FP64, true>{sv, j_vals, residual, qa, 0}, 8,
+                                    0, st);
             return launch_frame(plan, nullptr,
-                                FrameJac<MPK_PREC_FP32_LOCAL, false>{sv, j_vals, residual, qa, 0},
-                                0, 0, st);
+                                FrameJac<MPK_PREC_FP64, false>{sv, j_vals, residual, qa, 0}, 0, 0,
+                                st);
+        }
+        case MPK_PREC_FP32_LOCAL: {
+            StateView32 sv{};
+            if (int rc = elem_state(plan, P, tags, sol_a, nullptr, mat, st, &sv)) return rc;
+            using J1 = FrameJac<MPK_PREC_FP32_LOCAL, true, StateView32>;
+            using J0 = FrameJac<MPK_PREC_FP32_LOCAL, false, StateView32>;
+            if (calc_j) return launch_frame(plan, nullptr, J1{sv, j_vals, residual, qa, 0}, 8, 0, st);
+            return launch_frame(plan, nullptr, J0{sv, j_vals, residual, qa, 0}, 0, 0, st);
         }
     }
     set_error("unknown precision mode %d", precision);
@@ -358,28 +590,11 @@ int mpk_csc_newton_set(const mpk_plan* plan, const double* P, const int32_t* tag
                 "the fused Newton set needs the Nedelec P1 reference curls");
     MPK_REQUIRE(mat->num_tags >= 1, MPK_ERR_ARGS, "material table without tags");
     hipStream_t st = as_stream(stream);
-    StateView sv{};
-    if (int rc = elem_state(plan, P, tags, sol_a, sigma, mat, st, &sv)) return rc;
-    const bool with_m = m_vals != nullptr;
-    if (precision == MPK_PREC_FP64 && with_m)
-        return launch_frame(plan, nullptr,
-                            FrameNewton<MPK_PREC_FP64, true>{sv, k_vals, m_vals, j_vals, residual,
-                                                             qk, qm, 0, 0},
-                            3 * sizeof(double), 0, st);
     if (precision == MPK_PREC_FP64)
-        return launch_frame(plan, nullptr,
-                            FrameNewton<MPK_PREC_FP64, false>{sv, k_vals, nullptr, j_vals,
-                                                              residual, qk, qm, 0, 0},
-                            2 * sizeof(double), 0, st);
-    if (with_m)
-        return launch_frame(plan, nullptr,
-                            FrameNewton<MPK_PREC_FP32_LOCAL, true>{sv, k_vals, m_vals, j_vals,
-                                                                   residual, qk, qm, 0, 0},
-                            3 * sizeof(double), 0, st);
-    return launch_frame(plan, nullptr,
-                        FrameNewton<MPK_PREC_FP32_LOCAL, false>{sv, k_vals, nullptr, j_vals,
-                                                                residual, qk, qm, 0, 0},
-                        2 * sizeof(double), 0, st);
+        return newton_set<MPK_PREC_FP64, StateView>(plan, P, tags, sol_a, mat, sigma, qk, qm,
+                                                    k_vals, m_vals, j_vals, residual, st);
+    return newton_set<MPK_PREC_FP32_LOCAL, StateView32>(plan, P, tags, sol_a, mat, sigma, qk, qm,
+                                                        k_vals, m_vals, j_vals, residual, st);
 }
 
 }  // extern "C"

@@ -125,6 +125,14 @@ struct TileRange {
     int base, size;
 };
 template <class Op, class Acc>
+__device__ __forceinline__ void tile_zero(Op& op, Acc* acc, unsigned char* smem,
+                                          const TileRange& t) {
+    if constexpr (Planes<Op>::value > 1) op.stride = t.size;
+    op.block_init(smem + (size_t)op.acc_bytes_max);
+    for (int i = threadIdx.x; i < t.size * Planes<Op>::value; i += kW) acc[i] = Op::zero();
+    if constexpr (Planes<Op>::value > 0) __syncthreads();
+}
+template <class Op, class Acc>
 __device__ __forceinline__ TileRange tile_begin_at(const TileCtx& tc, Op& op, Acc* acc,
                                                    unsigned char* smem, int64_t tile_id) {
     TileRange t;
@@ -133,10 +141,7 @@ __device__ __forceinline__ TileRange tile_begin_at(const TileCtx& tc, Op& op, Ac
     t.jend = t.j0 + kW < tc.n ? t.j0 + kW : tc.n;
     t.base = tc.colptr[t.j0];
     t.size = tc.colptr[t.jend] - t.base;
-    if constexpr (Planes<Op>::value > 1) op.stride = t.size;
-    op.block_init(smem + (size_t)op.acc_bytes_max);
-    for (int i = threadIdx.x; i < t.size * Planes<Op>::value; i += kW) acc[i] = Op::zero();
-    if constexpr (Planes<Op>::value > 0) __syncthreads();
+    tile_zero(op, acc, smem, t);
     return t;
 }
 template <class Op, class Acc>

@@ -205,6 +205,7 @@ struct FrameMass {
 template <int kPrec>
 struct FrameKM {
     using Acc = double2;
+    static constexpr int kRing = 6;  // incidences per gather round (ring driver)
     using Ids = NoIds;
     using Elem = double2;  // (nu, sigma)
     struct Col {};
@@ -258,23 +259,46 @@ struct ElemState {
     double2 s1;  // dnu, sigma
 };
 struct StateView {
+    using Elem = ElemState;
+    using S0 = double4;
+    using S1 = double2;
     const double4* __restrict__ s0;
     const double2* __restrict__ s1;
-    __device__ ElemState load(uint32_t e) const { return ElemState{s0[e], s1[e]}; }
+    __device__ Elem load(uint32_t e) const { return Elem{s0[e], s1[e]}; }
+    __device__ double4 load_bnu(uint32_t e) const { return s0[e]; }
+};
+// FP32-local element state: the same fields rounded to FP32, 24 B per element
+// (half the gather and HBM bytes of the FP64 state), widened to FP64 at the
+// load so the arithmetic after it is the FP64-state code.
+struct StateView32 {
+    using Elem = ElemState;
+    using S0 = float4;
+    using S1 = float2;
+    const float4* __restrict__ s0;
+    const float2* __restrict__ s1;
+    __device__ Elem load(uint32_t e) const {
+        const float4 a = s0[e];
+        const float2 b = s1[e];
+        return Elem{make_double4(a.x, a.y, a.z, a.w), make_double2(b.x, b.y)};
+    }
+    __device__ double4 load_bnu(uint32_t e) const {
+        const float4 a = s0[e];
+        return make_double4(a.x, a.y, a.z, a.w);
+    }
 };
 
 // Newton Jacobian J = K(nu) + 2 dnu (B.C_r)(B.C_c) and residual R = K(nu) a
 // (assemble_jacobian_kernel_mex.cpp:94-131 per visit, in the frame).
-template <int kPrec, bool kCalcJ>
+template <int kPrec, bool kCalcJ, class SV = StateView>
 struct FrameJac {
     using Acc = double;
     using Ids = NoIds;
-    using Elem = ElemState;
+    using Elem = typename SV::Elem;
     struct Col {
         double r;
     };
     static constexpr int kPlanes = kCalcJ ? 1 : 0;  // residual only: no LDS tile
-    StateView st;
+    SV st;
     double* __restrict__ out;
     double* __restrict__ res;
     CscQuad qa;
@@ -320,17 +344,18 @@ struct FrameJac {
 // when kWithM, M: visits of linear elements (dnu = 0) touch only K and visits
 // of non-conducting ones (sigma = 0) skip M; J = K + D at the store.  Without
 // M the tile needs 16 B of LDS per entry instead of 24 (more waves per CU).
-template <int kPrec, bool kWithM>
+template <int kPrec, bool kWithM, class SV = StateView>
 struct FrameNewton {
     using Acc = double;
+    static constexpr int kRing = 6;  // incidences per gather round (ring driver)
     using Ids = NoIds;
-    using Elem = ElemState;
+    using Elem = typename SV::Elem;
     struct Col {
         double r;
     };
     static constexpr int kPlanes = kWithM ? 3 : 2;
     static constexpr int kD = kWithM ? 2 : 1;  // plane of D
-    StateView st;
+    SV st;
     double* __restrict__ out_k;
     double* __restrict__ out_m;
     double* __restrict__ out_j;
@@ -389,6 +414,61 @@ struct FrameNewton {
                 if (p != 0xff)
                     acc[kD * stride + p] += (two_inv * dot(Bv, g.u[rp])) * (js * fsign(f, rp));
             }
+        }
+    }
+};
+
+// The fused Newton set on a tile none of whose elements is conducting
+// (|sigma| <= 1e-12) or nonlinear at its B (dnu = 0): no M or D term reaches
+// such a tile, so it needs the K plane only (8 B of LDS per entry: about
+// three times the waves per CU of FrameNewton) and gathers (B, nu) alone.  The
+// store writes M = 0 and J = K + 0, the values FrameNewton stores for the
+// same tile.
+template <int kPrec, class SV>
+struct FrameNewtonLin {
+    using Acc = double;
+    static constexpr int kRing = 6;  // incidences per gather round (ring driver)
+    using Ids = NoIds;
+    using Elem = double4;  // Bx, By, Bz, nu
+    struct Col {
+        double r;
+    };
+    SV st;
+    double* __restrict__ out_k;
+    double* __restrict__ out_m;
+    double* __restrict__ out_j;
+    double* __restrict__ res;
+    CscQuad qk;
+    int acc_bytes_max;
+    __device__ static Acc zero() { return 0.0; }
+    __device__ void block_init(unsigned char*) const {}
+    __device__ Col col_begin() const { return Col{0.0}; }
+    __device__ void col_end(int64_t j, Col& cs) const {
+        if (res) res[j] = cs.r;
+    }
+    __device__ void store(int64_t p, Acc k) const {
+        if (out_k) out_k[p] = k;
+        if (out_m) out_m[p] = 0.0;
+        if (out_j) out_j[p] = k + 0.0;
+    }
+    __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
+    __device__ Elem elem(uint32_t e, const Ids&) const { return st.load_bnu(e); }
+    __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
+                            Acc* acc, Col& cs) const {
+        const FrameGeom g = frame_geom(nodes, false);
+        const Vec3 Bv{el.x, el.y, el.z};
+        const double nu = el.w;
+        const double two_inv = 2.0 / g.det;
+        const double inv = 1.0 / fabs(g.det);
+        const double wd = qk.wsum * fabs(g.det);
+        const double f0 = fsign(f, 0);
+        const double BC0 = two_inv * dot(Bv, g.u[0]);
+        cs.r += nu * BC0 * wd * f0;
+        const double ks = 4.0 * qk.wsum * nu * inv * f0;
+#pragma unroll
+        for (int rp = 0; rp < 6; ++rp) {
+            const int p = fpos(pos, rp);
+            if (p != 0xff) acc[p] += curl_dot<kPrec>(g, rp) * (ks * fsign(f, rp));
         }
     }
 };
@@ -476,32 +556,63 @@ __global__ void __launch_bounds__(kW) k_csc_frame(TileCtx tc, Op op) {
 // arithmetic of slots past the column's count is skipped.
 template <class Op, int kB>
 __device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restrict__ ends, Op& op,
-                                          unsigned char* smem, int64_t tile_id) {
+                                          unsigned char* smem, int64_t tile_id,
+                                          const int32_t* __restrict__ sel = nullptr,
+                                          int32_t want = 0) {
     using Acc = typename Op::Acc;
     using Ids = typename Op::Ids;
     using Elem = typename Op::Elem;
     Acc* acc = reinterpret_cast<Acc*>(smem);
-    const TileRange tr = tile_begin_at(tc, op, acc, smem, tile_id);
     const int lane = threadIdx.x;
+    TileRange tr;
+    tr.tile_id = tile_id;
+    tr.j0 = tile_id * kW;
+    tr.jend = tr.j0 + kW < tc.n ? tr.j0 + kW : tc.n;
     const int64_t j = tr.j0 + lane;
-    if (j < tr.jend) {
-        const int cbase = tc.colptr[j];
-        Acc* col = acc + (cbase - tr.base);
-        typename Op::Col cs = op.col_begin();
-        const int cnt = tc.inc_ptr[j + 1] - tc.inc_ptr[j];
-        if (cnt > 0) {
-            const uint4* __restrict__ rec = tc.inc_ring + tc.frame_base[tr.tile_id] + lane;
-            const int2 lh = ends[j];
-            const Vec3 xlo = ld_node(tc.P4, lh.x), xhi = ld_node(tc.P4, lh.y);
-            Vec3 xprev;
-            for (int k0 = 0; k0 < cnt; k0 += kB) {
-                uint4 r[kB];
+    const int64_t jc = j < tr.jend ? j : tr.jend - 1;  // a valid column for every lane
+    // Every index of the tile depends on the tile and the lane only: the
+    // loads go out together, one memory latency before the records instead of
+    // a chain of dependent ones.  A tile of another class is not branched
+    // around (the compiler would hoist that branch above the loads): it runs
+    // with no live lanes and an empty range, so it writes nothing.
+    const int cbase = tc.colptr[jc];
+    const int inc0 = tc.inc_ptr[jc];
+    const int inc1 = tc.inc_ptr[jc + 1];
+    const int2 lh = ends[jc];
+    const int base = tc.colptr[tr.j0];
+    const int top = tc.colptr[tr.jend];
+    const int fbase = tc.frame_base[tile_id];
+    const int32_t cls = sel ? sel[tile_id] : want;
+    const bool mine = cls == want;
+    const bool live = mine && j < tr.jend;
+    tr.base = base;
+    tr.size = mine ? top - base : 0;
+    // The first gather round's records go out before the LDS accumulator is
+    // cleared, for every lane: load indices are clamped to the column's last
+    // record, and to record row 0 when the lane has none (every column of a
+    // plan has at least one incidence, so each tile's record block has at
+    // least one row of kW records; a lane without records never uses them).
+    const int cnt = live ? inc1 - inc0 : 0;
+    const int klast = cnt > 0 ? cnt - 1 : 0;
+    const uint4* __restrict__ rec = tc.inc_ring + fbase + lane;
+    uint4 r[kB];
 #pragma unroll
-                for (int i = 0; i < kB; ++i) {
-                    const int k = k0 + i < cnt ? k0 + i : cnt - 1;
-                    r[i] = rec[k * kW];
+    for (int i = 0; i < kB; ++i) r[i] = rec[(i < klast ? i : klast) * kW];
+    tile_zero(op, acc, smem, tr);
+    if (live) {
+        Acc* col = acc + (cbase - base);
+        typename Op::Col cs = op.col_begin();
+        if (cnt > 0) {
+            const Vec3 xlo = ld_node(tc.P4, lh.x), xhi = ld_node(tc.P4, lh.y);
+            Vec3 xprev = ld_node(tc.P4, rrec_start(r[0]));
+            for (int k0 = 0; k0 < cnt; k0 += kB) {
+                if (k0 > 0) {
+#pragma unroll
+                    for (int i = 0; i < kB; ++i) {
+                        const int k = k0 + i < cnt ? k0 + i : cnt - 1;
+                        r[i] = rec[k * kW];
+                    }
                 }
-                if (k0 == 0) xprev = ld_node(tc.P4, rrec_start(r[0]));
                 Ids d[kB];
 #pragma unroll
                 for (int i = 0; i < kB; ++i) d[i] = op.ids(tc, cbase, rrec_pos(r[i]));
@@ -527,11 +638,33 @@ __device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restr
     tile_end(op, tr, acc);
 }
 
+// sel (optional): per-tile class flags; a block whose tile's flag differs
+// from `want` returns at once (the tiles of a plan split over two launches).
 template <class Op, int kB>
 __global__ void __launch_bounds__(kW) k_csc_ring_batch(TileCtx tc, const int2* __restrict__ ends,
-                                                       Op op) {
+                                                       Op op, const int32_t* __restrict__ sel,
+                                                       int32_t want) {
     extern __shared__ __align__(16) unsigned char smem[];
-    ring_tile<Op, kB>(tc, ends, op, smem, xcd_tile(blockIdx.x, gridDim.x));
+    ring_tile<Op, kB>(tc, ends, op, smem, xcd_tile(blockIdx.x, gridDim.x), sel, want);
+}
+
+// Tiles from a device list (persistent grid): block b runs on XCD b % 8; the
+// blocks of XCD x walk the contiguous list range [x c / 8, (x+1) c / 8),
+// c = *count, so neighbouring tiles of the list share that XCD's L2.
+template <class Op, int kB>
+__global__ void __launch_bounds__(kW) k_csc_ring_list(TileCtx tc, const int2* __restrict__ ends,
+                                                      Op op, const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ count) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int64_t kXcd = 8;
+    const int64_t c = *count;
+    const int64_t x = blockIdx.x % kXcd;
+    const int64_t nbx = ((int64_t)gridDim.x - x + kXcd - 1) / kXcd;  // blocks on XCD x
+    const int64_t lo = c * x / kXcd, hi = c * (x + 1) / kXcd;
+    for (int64_t p = lo + blockIdx.x / kXcd; p < hi; p += nbx) {
+        ring_tile<Op, kB>(tc, ends, op, smem, list[p]);
+        __syncthreads();
+    }
 }
 
 // K + j w_f M_sigma for several frequencies from one pass: the LDS tile holds

@@ -28,6 +28,12 @@ struct mpk_plan {
     // double4 (Bx, By, Bz, nu) at state, then double2 (dnu/d|B|^2, sigma) at
     // state + 4*ne.
     mutable mpk::DevBuf buf_state;
+    // Tile classes of the fused Newton set, written with the element state,
+    // [2 ntiles + 1] int32 (ntiles = column tiles of kTileCols columns):
+    // flag per tile (1 if an element with an M or D term touches the tile),
+    // the list of flagged tiles, their count.  Allocated on first use.
+    mutable mpk::DevBuf buf_tile_flag;
+    mutable int num_cus = 0;  // compute units of the plan's device (queried on first use)
 
     // Closed-form (frame) kernels: one 16-byte record per incidence.
     //   x = element id

@@ -380,11 +380,29 @@ def test_csc_newton_set(cfg1, prec, driver):
                                        curl), n, n)
     Mref = O.sparse_scipy(*O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sig, qm, val),
                           n, n)
+    # the ring driver splits the 64-column tiles into those an M or D term
+    # reaches (an element with sigma != 0 or dnu != 0 among the tile's
+    # incidences) and the rest; this workload has both kinds
+    _, dnu_e = O.eval_nu((B * B).sum(axis=1), mesh.RegionTags, packed)
+    hot = (sig != 0) | (dnu_e != 0)
+    dofs = mi["Dofs"].astype(np.int64)[hot] - 1  # Ne x 6, 0-based, -1 inactive
+    hot_tiles = np.unique(dofs[dofs >= 0] // 64)
+    ntiles = (n + 63) // 64
+    assert 0 < hot_tiles.size < ntiles
     tol = TOL[prec]
     for name, ref in (("K", Kref), ("M", Mref), ("J", Jref)):
         A = csc_from_plan(plan, out[name])
         assert abs(A - ref).max() / abs(ref).max() < tol, name
     assert rel_err(out["R"].cpu().numpy(), Rref) < tol
+    # tiles no M or D term reaches: M exactly 0 and J exactly K there
+    colptr = plan.pattern()[0].cpu().numpy().astype(np.int64)
+    cold = np.ones(n, dtype=bool)
+    for t in hot_tiles:
+        cold[64 * t: 64 * (t + 1)] = False
+    sel = np.repeat(cold, np.diff(colptr))
+    Mv, Jv, Kv = (out[k].cpu().numpy() for k in ("M", "J", "K"))
+    assert sel.any() and not Mv[sel].any()
+    assert np.array_equal(Jv[sel], Kv[sel])
     # subsets of outputs give the same values
     only = asm.assembleNewtonSet(space, a, lib, sigmaMap=sig, precision=prec, outputs=("J",))
     assert torch_equal_bits(only["J"], out["J"])
