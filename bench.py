@@ -260,7 +260,7 @@ def run_ecore(args, emit=True):
         if kind == "learned":
             w = 2 * np.pi * freq
             pred = NuPredictor(asm, lib, seed=0, drive=lambda t: (np.sin(w * t), np.cos(w * t)))
-            hook = learned_warm_start(pred, space)
+            hook = learned_warm_start(pred, space, eta_warm=args.warm_eta)
             solver.InitialGuess = "extrapolated"  # extrapolation stays the fallback
             bcache = {}
 
@@ -536,6 +536,8 @@ def main():
                     help="ecore: cubes per axis (64 64 40: 983,040 tets)")
     ap.add_argument("--batch-meshes", type=int, default=256, help="batch: number of meshes")
     ap.add_argument("--batch-freqs", type=int, default=32, help="batch: number of frequencies")
+    ap.add_argument("--warm-eta", type=float, default=1e-2,
+                    help="ecore: relative tolerance of the learned warm start's solve")
     ap.add_argument("--no-nonlinear", action="store_true",
                     help="freq128: skip the short learned-vs-plain nonlinear transient")
     ap.add_argument("--raw-source", action="store_true",

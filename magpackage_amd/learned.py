@@ -203,7 +203,9 @@ def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5,
     RelTolerance |R(x_prev)|) absolute): Newton polishes what the prediction
     and the solve leave.  Tighter warm solves do not pay: even with nu_hat
     within 0.05% the warm residual stays near 2e-3 of the step's first
-    residual, and Newton takes the same number of iterations."""
+    residual, and Newton takes the same number of iterations.  Looser ones do
+    not either: a step's first solve needs ~400 PCG iterations for any
+    tolerance from 1e-2 to 0.3 on the E-core transient."""
     asm = predictor.asm
 
     def hook(ctx):
