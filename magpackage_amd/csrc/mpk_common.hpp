@@ -87,6 +87,32 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// A library-owned second stream with a fork and a join event, for calls that
+// run two independent kernels side by side: the caller's stream records
+// `fork`, the side stream waits on it, runs its kernel and records `join`, and
+// the caller's stream waits on `join` before anything after the call.  The
+// stream has the default priority.  Created on first use on the current
+// device; calls that use one object must not overlap.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    SideStream() = default;
+    SideStream(const SideStream&) = delete;
+    SideStream& operator=(const SideStream&) = delete;
+    ~SideStream() {
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    hipError_t ensure() {
+        hipError_t e = hipSuccess;
+        if (!s) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e == hipSuccess && !fork) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+        if (e == hipSuccess && !join) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+        return e;
+    }
+};
+
 // Device view of the packed material tables (MexMaterialUtils.hpp:9-18).
 struct MaterialView {
     const double* lin;

@@ -287,7 +287,7 @@ bool ring_driver(const mpk_plan* plan) {
 // (list[0, *count)): as many blocks as fit on the device at the op's LDS size.
 template <class Op>
 int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t st,
-                     const int32_t* list, const int32_t* count) {
+                     const int32_t* list, const int32_t* count, size_t max_per_cu = 16) {
     size_t lds = 0;
     if (int rc = lds_bytes(plan, op, acc_size, 0, &lds)) return rc;
     if (plan->num_cus == 0) {
@@ -298,7 +298,7 @@ int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t s
     }
     constexpr size_t kLdsPerCu = 160 * 1024;
     size_t per_cu = lds ? kLdsPerCu / lds : 16;
-    per_cu = per_cu < 1 ? 1 : (per_cu > 16 ? 16 : per_cu);
+    per_cu = per_cu < 1 ? 1 : (per_cu > max_per_cu ? max_per_cu : per_cu);
     const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
     // the kernel splits the list over the 8 XCDs by block index: the grid is a
     // multiple of 8 so that every XCD's range has blocks
@@ -377,6 +377,10 @@ int launch_frame(const mpk_plan* plan, const double* P, Op op, size_t acc_size, 
     return MPK_OK;
 }
 
+// Persistent blocks per CU of the Newton set's listed pass when it runs beside
+// the class-0 pass.
+constexpr size_t kSidePerCu = 2;
+
 // The fused Newton set.  Ring driver: the element-state kernel also classes
 // the column tiles; tiles that no M or D term reaches run FrameNewtonLin (one
 // LDS plane), the others FrameNewton (two or three planes), in two launches.
@@ -404,22 +408,60 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
                             with_m || sigma != nullptr, true))
         return rc;
     if (split) {
-        if (int rc = launch_frame(plan, nullptr,
-                                  FrameNewtonLin<kPrec, SV>{sv, k_vals, m_vals, j_vals, residual,
-                                                            qk, 0},
-                                  sizeof(double), 0, st, flags, 0))
-            return rc;
         const int32_t* list = flags + ntiles;
         const int32_t* count = flags + 2 * ntiles;
-        if (with_m)
-            return launch_ring_list(plan,
-                                    FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals, j_vals,
-                                                                 residual, qk, qm, 0, 0},
-                                    3 * sizeof(double), st, list, count);
-        return launch_ring_list(plan,
-                                FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr, j_vals,
-                                                              residual, qk, qm, 0, 0},
-                                2 * sizeof(double), st, list, count);
+        // Schedule, by measurement on the 4M-tet Newton workload: with the
+        // FP64 state the listed pass runs best beside the class-0 pass, on the
+        // plan's side stream with a few persistent blocks per CU (0.90 -> 0.84
+        // ms); with the FP32 state, after it with the full persistent grid
+        // (0.81 ms; side by side 0.84-1.07).  MPK_NEWTON_OVERLAP=0/1 and
+        // MPK_LIST_BLOCKS_PER_CU (1..16) override.
+        bool overlap = kPrec == MPK_PREC_FP64;
+        if (const char* e = std::getenv("MPK_NEWTON_OVERLAP")) overlap = std::atoi(e) != 0;
+        size_t per_cu = overlap ? kSidePerCu : 16;
+        if (const char* e = std::getenv("MPK_LIST_BLOCKS_PER_CU")) {
+            const int v = std::atoi(e);
+            if (v >= 1 && v <= 16) per_cu = (size_t)v;
+        }
+        auto listed = [&](hipStream_t s) {
+            return with_m ? launch_ring_list(plan,
+                                             FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals,
+                                                                          j_vals, residual, qk,
+                                                                          qm, 0, 0},
+                                             3 * sizeof(double), s, list, count, per_cu)
+                          : launch_ring_list(plan,
+                                             FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr,
+                                                                           j_vals, residual, qk,
+                                                                           qm, 0, 0},
+                                             2 * sizeof(double), s, list, count, per_cu);
+        };
+        auto class0 = [&]() {
+            return launch_frame(plan, nullptr,
+                                FrameNewtonLin<kPrec, SV>{sv, k_vals, m_vals, j_vals, residual, qk,
+                                                          0},
+                                sizeof(double), 0, st, flags, 0);
+        };
+        if (!overlap) {
+            if (int rc = class0()) return rc;
+            return listed(st);
+        }
+        // the listed tiles on the side stream, the class-0 pass on the
+        // caller's stream; the caller's stream waits for both before anything
+        // after the call
+        MPK_HIP_TRY(plan->side.ensure());
+        hipStream_t s2 = plan->side.s;
+        MPK_HIP_TRY(hipEventRecord(plan->side.fork, st));
+        MPK_HIP_TRY(hipStreamWaitEvent(s2, plan->side.fork, 0));
+        const int rc = listed(s2);
+        const hipError_t ej = hipEventRecord(plan->side.join, s2);
+        const int rc0 = class0();
+        // join after the class-0 launch, so the two passes overlap
+        const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(st, plan->side.join, 0) : ej;
+        if (rc) return rc;
+        if (rc0) return rc0;
+        MPK_HIP_TRY(ej);
+        MPK_HIP_TRY(ew);
+        return MPK_OK;
     }
     if (with_m)
         return launch_frame(plan, nullptr,

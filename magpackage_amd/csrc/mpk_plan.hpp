@@ -34,6 +34,8 @@ struct mpk_plan {
     // the list of flagged tiles, their count.  Allocated on first use.
     mutable mpk::DevBuf buf_tile_flag;
     mutable int num_cus = 0;  // compute units of the plan's device (queried on first use)
+    // side stream of the fused Newton set (its two tile classes run side by side)
+    mutable mpk::SideStream side;
 
     // Closed-form (frame) kernels: one 16-byte record per incidence.
     //   x = element id

@@ -410,6 +410,76 @@ def test_csc_newton_set(cfg1, prec, driver):
         asm.assembleNewtonSet(space, a, lib, precision="fp64_exact")
 
 
+def _newton_refs(mesh, mi, lib, a, sig):
+    """Oracle K(nu(B)), M_sigma, J, R for the fused Newton set."""
+    packed = pack_material_data(lib)
+    qw, curl, _ = stiffness_tables()
+    qm, val, _ = mass_tables(2)
+    n = mi["ndof"]
+    tags = mesh.RegionTags.astype(np.float64)
+    I, J, V, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, a, qw, curl,
+                                 packed)
+    Rref = np.zeros(n)
+    np.add.at(Rref, Ri.astype(np.int64) - 1, Rv)
+    B = element_B(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a)
+    nu_e, _ = O.eval_nu((B * B).sum(axis=1), mesh.RegionTags, packed)
+    return {"K": O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu_e, qw,
+                                             curl), n, n),
+            "M": O.sparse_scipy(*O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sig, qm, val),
+                                n, n),
+            "J": O.sparse_scipy(I, J, V, n, n), "R": Rref}
+
+
+@pytest.mark.parametrize("overlap", ["0", "1"])
+@pytest.mark.parametrize("case", ["all_linear", "all_sigma", "tiny"])
+def test_newton_set_tile_classes(gpu, case, overlap, monkeypatch):
+    """The ring driver runs tiles that no M or D term reaches with a one-plane
+    op and lists the others for a persistent pass, after it or beside it on a
+    second stream (overlap).  Edge cases: no listed tile (linear,
+    non-conducting), every tile listed (sigma everywhere), and a mesh of 2
+    tiles (fewer than the 8 XCDs the persistent grid is split over).  Against
+    the oracle and against the element-order driver, which runs every tile
+    with the three-plane op."""
+    monkeypatch.setenv("MPK_NEWTON_OVERLAP", overlap)
+    from magpackage_amd.meshgen import tag_box_region
+    if case == "tiny":
+        mesh = kuhn_box(2, 2, 2, seed=7)
+        tag_box_region(mesh, (0.0, 0.0, 0.0), (0.5, 1.0, 1.0), 2)
+    else:
+        mesh = config1_mesh(10, seed=5)
+    M_, dh, space, asm = _setup(mesh, gpu)
+    mi = O.mesh_inputs(mesh)
+    n = mi["ndof"]
+    if case == "all_linear":
+        lib = {1: create_linear(1.0), 2: create_linear(1000.0)}
+        sig = np.zeros(mesh.num_elems)
+    elif case == "all_sigma":
+        lib = {1: create_linear(1.0), 2: marrocco()}
+        sig = np.full(mesh.num_elems, 3e6)
+    else:
+        lib = {1: create_linear(1.0), 2: marrocco()}
+        sig = np.where(mesh.RegionTags == 2, 2e6, 0.0)
+    a = np.random.default_rng(21).normal(size=n) * 0.004
+    plan = asm.plan(space)
+    if case == "tiny":
+        assert (n + 63) // 64 < 8 and plan.ring_order
+    monkeypatch.setenv("MPK_CSC_DRIVER", "ring")
+    ring = asm.assembleNewtonSet(space, a, lib, sigmaMap=sig, precision="fp64")
+    monkeypatch.setenv("MPK_CSC_DRIVER", "element")
+    elem = asm.assembleNewtonSet(space, a, lib, sigmaMap=sig, precision="fp64")
+    refs = _newton_refs(mesh, mi, lib, a, sig)
+    for name in ("K", "M", "J"):
+        A = csc_from_plan(plan, ring[name])
+        scale = max(abs(refs[name]).max(), 1e-300)
+        assert abs(A - refs[name]).max() / scale < TOL["fp64"], name
+        d = (ring[name] - elem[name]).abs().max().item()
+        assert d <= 1e-13 * max(elem[name].abs().max().item(), 1e-300), name
+    assert rel_err(ring["R"].cpu().numpy(), refs["R"]) < TOL["fp64"]
+    if case == "all_linear":
+        assert not ring["M"].cpu().numpy().any()
+        assert np.array_equal(ring["J"].cpu().numpy(), ring["K"].cpu().numpy())
+
+
 def torch_equal_bits(a, b):
     import torch
     return torch.equal(a.view(torch.int64), b.view(torch.int64))
