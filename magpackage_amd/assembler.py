@@ -77,6 +77,19 @@ class Mesh:
     def from_kuhn(km) -> "Mesh":
         return Mesh(km.P, km.T, km.RegionTags)
 
+    @staticmethod
+    def load(filename: str, unit_tag: str = "m") -> "Mesh":
+        """Mesh.load (src/Mesh.m:29-63): a COMSOL .mphtxt mesh, coordinates
+        scaled to metres.  Gmsh .msh input (read_msh) is not supported."""
+        from .mesh_io import read_mphtxt
+        if str(filename).lower().endswith(".mphtxt"):
+            raw = read_mphtxt(str(filename), unit_tag)
+        elif str(filename).lower().endswith(".msh"):
+            raise NotImplementedError("Gmsh .msh meshes are not supported; use .mphtxt")
+        else:
+            raise ValueError(f"unsupported mesh format: {filename}")
+        return Mesh(raw.P, raw.T, raw.RegionTags)
+
     def generateEdges(self) -> None:
         """Mesh.generateEdges (src/Mesh.m:114-138) on the GPU."""
         if self.Edges is not None:

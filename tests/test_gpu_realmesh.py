@@ -1,0 +1,194 @@
+"""GPU parity on the reference's own COMSOL meshes (tests/golden/ecore_mesh.npz,
+team7_mesh.npz, read from data/Ecore.mphtxt and data/Team7.mphtxt): unstructured
+tetrahedra with up to 9 elements around an edge, where the Kuhn boxes of the
+other tests have at most 6 (one gather round of the ring driver).
+
+Bar as in test_gpu_parity: bit-exact DOF map, sparsity pattern, MEX-format
+COO output and FP64_EXACT matrices; <= 1e-12 (FP64) and <= 1e-6 (FP32_LOCAL)
+relative otherwise.  The E-core's iron (region 2) uses the B-H table of the
+reference's E-core tutorial (tutorials/run_ecore_transient.m:55-60), built
+into a PCHIP reluctivity curve as MaterialLib.createNonlinear does.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from magpackage_amd.material import create_linear, create_nonlinear, pack_material_data
+from tests.helpers import csc_equal_bits, element_B, mass_tables, rel_err, stiffness_tables
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+TOL = {"fp64": 1e-12, "fp32_local": 1e-6}
+
+# tutorials/run_ecore_transient.m:55-60
+B_DATA = [0, 0.1, 0.2, 0.3001, 0.4, 0.5001, 0.6001, 0.7, 0.8, 0.9001, 1, 1.1001, 1.2001, 1.3002,
+          1.4, 1.4999, 1.5999, 1.6991, 1.7987, 1.8978, 1.98, 1.9925, 2.118164]
+H_DATA = [0, 7.86, 13.99, 19.51, 24.58, 29.42, 34.07, 38.64, 43.12, 47.59, 52.1, 56.77, 61.67,
+          67.19, 74.03, 83.86, 1.00e2, 1.38e2, 2.95e2, 1.01e3, 2500, 12500, 112500]
+
+
+class _Raw:
+    def __init__(self, d):
+        self.P = d["P"]
+        self.T = d["T"].astype(np.int64)
+        self.RegionTags = d["tags"].astype(np.int32)
+        self.num_elems = self.T.shape[0]
+
+
+@pytest.fixture(scope="module", params=["ecore", "team7"])
+def real(request, gpu):
+    from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
+    raw = _Raw(np.load(GOLDEN / f"{request.param}_mesh.npz"))
+    M = Mesh(raw.P, raw.T, raw.RegionTags)
+    dh = DofHandler(M)
+    space = FunctionSpace("Nedelec", 1)
+    dh.distributeDofs(space)
+    asm = Assembler(M, dh)
+    mi = O.mesh_inputs(raw)
+    ntag = int(raw.RegionTags.max())
+    lib = {t: create_linear(1.0) for t in range(1, ntag + 1)}
+    lib[2] = create_nonlinear(B_DATA, H_DATA)
+    nu = np.where(raw.RegionTags == 2, 1.0 / (1000.0 * 4e-7 * np.pi), 1.0 / (4e-7 * np.pi))
+    sigma = np.where(raw.RegionTags == 2, 2e6, np.where(raw.RegionTags >= 3, 5.8e7, 0.0))
+    # A ~ N(0,1) (seed 11) scaled so that max |B| over the iron is 2.2 T
+    a = np.random.default_rng(11).normal(size=mi["ndof"])
+    B = element_B(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a)
+    a *= 2.2 / np.linalg.norm(B[raw.RegionTags == 2], axis=1).max()
+    return dict(name=request.param, raw=raw, M=M, dh=dh, space=space, asm=asm, mi=mi, lib=lib,
+                nu=nu, sigma=sigma, a=a)
+
+
+@pytest.fixture(params=["ring", "element"])
+def driver(request, monkeypatch):
+    monkeypatch.setenv("MPK_CSC_DRIVER", request.param)
+    return request.param
+
+
+def test_real_edges_dofs_pattern(real):
+    M, dh, mi, asm, space = real["M"], real["dh"], real["mi"], real["asm"], real["space"]
+    M.generateEdges()
+    assert np.array_equal(M.Edges.cpu().numpy(), mi["edges"])
+    assert np.array_equal(M.T2E.cpu().numpy(), mi["t2e"])
+    assert np.array_equal(M.T2E_Sign.cpu().numpy(), mi["sign"])
+    assert np.array_equal(dh.DofMaps["Nedelec_P1"].cpu().numpy(), mi["cell"])
+    plan = asm.plan(space)
+    qw, curl, _ = stiffness_tables()
+    I, J, V = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"],
+                          np.ones(real["raw"].num_elems), qw, curl)
+    c, r, _ = O.sparse(I, J, V, mi["ndof"], mi["ndof"], drop_zeros=False)
+    colptr, rowind = plan.pattern()
+    assert np.array_equal(colptr.cpu().numpy(), c)
+    assert np.array_equal(rowind.cpu().numpy(), r)
+    # the ring records need every edge's elements to form one ring or chain
+    # (a manifold neighbourhood); both meshes are
+    assert plan.ring_order
+    # edges with more incident elements than one 6-incidence gather round holds
+    assert np.bincount(mi["t2e"].ravel()).max() > 6
+
+
+def test_real_coo_mex_bit_exact(real):
+    from magpackage_amd import mex
+    mi, raw = real["mi"], real["raw"]
+    qw, curl, _ = stiffness_tables()
+    qm, val, _ = mass_tables(2)
+    args = (mi["P"].T, mi["T"].T, mi["Dofs"].T, mi["Signs"].T)
+    I, J, V = mex.assemble_curl_curl_kernel_mex(*args, real["nu"], qw, curl)
+    Io, Jo, Vo = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], real["nu"], qw, curl)
+    assert np.array_equal(I, Io) and np.array_equal(J, Jo)
+    assert np.array_equal(V.view(np.uint64), Vo.view(np.uint64))
+    I, J, V = mex.assemble_mass_kernel_mex(*args, real["sigma"], qm, val)
+    Io, Jo, Vo = O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], real["sigma"], qm, val)
+    assert np.array_equal(V.view(np.uint64), Vo.view(np.uint64))
+    packed = pack_material_data(real["lib"])
+    tags = raw.RegionTags.astype(np.float64)
+    out = mex.assemble_jacobian_kernel_mex(*args, tags, real["a"], qw, curl, *[
+        packed[k] for k in ("linNu", "isNon", "maxB", "breaks", "coefs", "starts", "counts",
+                            "coe_starts")], 1.0)
+    ref = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, real["a"], qw, curl, packed)
+    for g, r in zip(out, ref):
+        assert np.array_equal(np.asarray(g).view(np.uint64), r.view(np.uint64))
+
+
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
+def test_real_stiffness_mass(real, prec, driver):
+    if prec == "fp64_exact" and driver != "ring":
+        pytest.skip("the exact mode has a single (element-order) driver")
+    mi, asm, space = real["mi"], real["asm"], real["space"]
+    n = mi["ndof"]
+    qw, curl, _ = stiffness_tables()
+    qm, val, _ = mass_tables(2)
+    Kref = O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], real["nu"], qw,
+                                       curl), n, n)
+    Mref = O.sparse_scipy(*O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], real["sigma"], qm,
+                                  val), n, n)
+    K = asm.assembleStiffness(space, real["nu"], precision=prec, drop_zeros=False).to_scipy()
+    M = asm.assembleMassWeighted(space, real["sigma"], precision=prec,
+                                 drop_zeros=False).to_scipy()
+    if prec == "fp64_exact":
+        assert csc_equal_bits(asm.assembleStiffness(space, real["nu"], precision=prec).to_scipy(),
+                              Kref)
+        assert csc_equal_bits(asm.assembleMassWeighted(space, real["sigma"],
+                                                       precision=prec).to_scipy(), Mref)
+    else:
+        assert abs(K - Kref).max() / abs(Kref).max() < TOL[prec]
+        assert abs(M - Mref).max() / abs(Mref).max() < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
+def test_real_jacobian(real, prec, driver):
+    if prec == "fp64_exact" and driver != "ring":
+        pytest.skip("the exact mode has a single (element-order) driver")
+    mi, asm, space, raw = real["mi"], real["asm"], real["space"], real["raw"]
+    n = mi["ndof"]
+    packed = pack_material_data(real["lib"])
+    qw, curl, _ = stiffness_tables()
+    tags = raw.RegionTags.astype(np.float64)
+    I, J, V, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, real["a"], qw,
+                                 curl, packed)
+    Jref = O.sparse_scipy(I, J, V, n, n)
+    c, r, v = O.sparse(Ri, np.ones_like(Ri), Rv, n, 1, drop_zeros=False)
+    Rref = np.zeros(n)
+    Rref[r] = v
+    Jg, Rg = asm.assembleJacobian(space, real["a"], real["lib"], calcJ=True, precision=prec)
+    Rg = Rg.cpu().numpy()
+    if prec == "fp64_exact":
+        assert csc_equal_bits(Jg.to_scipy(), Jref)
+        assert np.array_equal(Rg.view(np.uint64), Rref.view(np.uint64))
+    else:
+        assert abs(Jg.to_scipy() - Jref).max() / abs(Jref).max() < TOL[prec]
+        assert rel_err(Rg, Rref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
+def test_real_newton_set(real, prec, driver):
+    """Fused K(nu(B)) + M_sigma + J + R against the oracle's separate kernels."""
+    mi, asm, space, raw = real["mi"], real["asm"], real["space"], real["raw"]
+    n = mi["ndof"]
+    packed = pack_material_data(real["lib"])
+    qw, curl, _ = stiffness_tables()
+    qm, val, _ = mass_tables(2)
+    tags = raw.RegionTags.astype(np.float64)
+    a, sig = real["a"], real["sigma"]
+    out = asm.assembleNewtonSet(space, a, real["lib"], sigmaMap=sig, precision=prec)
+    plan = asm.plan(space)
+    I, J, V, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, a, qw, curl,
+                                 packed)
+    Rref = np.zeros(n)
+    np.add.at(Rref, Ri.astype(np.int64) - 1, Rv)
+    B = element_B(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a)
+    nu_e, _ = O.eval_nu((B * B).sum(axis=1), raw.RegionTags, packed)
+    refs = {"K": O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu_e, qw,
+                                             curl), n, n),
+            "M": O.sparse_scipy(*O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sig, qm, val),
+                                n, n),
+            "J": O.sparse_scipy(I, J, V, n, n)}
+    import scipy.sparse as sp
+    colptr, rowind = plan.pattern()
+    for name, ref in refs.items():
+        A = sp.csc_matrix((out[name].cpu().numpy(), rowind.cpu().numpy(), colptr.cpu().numpy()),
+                          shape=(n, n))
+        assert abs(A - ref).max() / abs(ref).max() < TOL[prec], name
+    assert rel_err(out["R"].cpu().numpy(), Rref) < TOL[prec]
