@@ -119,6 +119,42 @@ int mpk_source_coo(const double* P, int64_t nn, const double* T, int64_t ne,
                    const double* cell_dofs, const double* signs, const double* Js,
                    const mpk_quadrature* quad, double* I_out, double* V_out, void* stream);
 
+/* Replaces assemble_scalar_laplacian_kernel_mex
+ * (src/mex/assemble_scalar_laplacian_kernel_mex.cpp:13-105):
+ * K = (sigma grad phi_i, grad phi_j) over P1 nodal functions.  node_dofs
+ * [4 x ne] (1-based, doubles), coeffs [ne] sigma (|sigma| <= 1e-12: zero
+ * block, indices still written); quad carries Qw (its tables are unused);
+ * grad_ref [4 x 3 x nq] host array at i + 4c + 12q.  Outputs 16*ne triplets,
+ * slot 16*e + 4*c + r = (dofs[r], dofs[c], K[r][c]).  Device arrays except
+ * quad and grad_ref.  Synchronises `stream` for the node-index check. */
+int mpk_scalar_laplacian_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                             const double* node_dofs, const double* coeffs,
+                             const mpk_quadrature* quad, const double* grad_ref,
+                             double* I_out, double* J_out, double* V_out, void* stream);
+
+/* Replaces assemble_coupling_kernel_mex (src/mex/assemble_coupling_kernel_mex.cpp:13-131):
+ * the A-V coupling block C = (sigma N_i, grad phi_j), 6 edge rows x 4 node
+ * columns.  edge_dofs / signs [6 x ne], node_dofs [4 x ne], coeffs [ne];
+ * quad->ref_val is the Nedelec table NedRef, grad_ref the [4 x 3 x nq] host
+ * table.  Outputs 24*ne triplets, slot 24*e + 6*c + r =
+ * (edge_dofs[r], node_dofs[c], C[r][c] * signs[r]).  Synchronises `stream`. */
+int mpk_coupling_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                     const double* edge_dofs, const double* signs, const double* node_dofs,
+                     const double* coeffs, const mpk_quadrature* quad, const double* grad_ref,
+                     double* I_out, double* J_out, double* V_out, void* stream);
+
+/* Replaces assemble_winding_kernel_mex (src/mex/assemble_winding_kernel_mex.cpp:13-126):
+ * F_i = (scale * J_dir, N_i) over the elements with (int)tags[e] == (int)coil_id;
+ * other elements give (0, 0) pairs.  J_dir from dir_field [3 x ne] (device, may be
+ * NULL) else from uniform_dir (host, 3 doubles).  quad carries Qw and ValRef.
+ * Outputs 6*ne pairs, slot 6*e + i = (dofs[i], F_i * signs[i]).  Synchronises
+ * `stream` for the node-index check. */
+int mpk_winding_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                    const double* cell_dofs, const double* signs, const double* tags,
+                    double coil_id, const double* dir_field, const double* uniform_dir,
+                    double scale, const mpk_quadrature* quad, double* I_out, double* V_out,
+                    void* stream);
+
 /* PostProcessor.computeElementB (src/post/PostProcessor.m:64-126) for one
  * solution column: B_out [3 x ne] device, B_e = curl A at the element.  T
  * [ne x 4] and cell_dofs [ne x 6] device int32, 1-based (0 = inactive DOF);

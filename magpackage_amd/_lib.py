@@ -83,6 +83,12 @@ _SIGS = {
     "mpk_jacobian_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _pq, _pm, _i32,
                           _vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "mpk_source_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _pq, _vp, _vp, _vp], _i32),
+    "mpk_scalar_laplacian_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _pq, _vp, _vp, _vp, _vp,
+                                  _vp], _i32),
+    "mpk_coupling_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _pq, _vp, _vp, _vp, _vp,
+                          _vp], _i32),
+    "mpk_winding_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, C.c_double, _vp, _vp, C.c_double,
+                         _pq, _vp, _vp, _vp], _i32),
     "mpk_element_b": ([_vp, _i64, _vp, _vp, _i64, _vp, _i64, _pq, _vp, _vp], _i32),
     "mpk_csc_source": ([_plan_p, _vp, _vp, _pq, _vp, _vp], _i32),
     "mpk_eval_nu": ([_vp, _vp, _i64, _pm, _vp, _vp, _vp], _i32),

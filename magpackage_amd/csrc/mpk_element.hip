@@ -10,6 +10,9 @@
 //   k_curl_curl_coo  src/mex/assemble_curl_curl_kernel_mex.cpp:44-121
 //   k_mass_coo       src/mex/assemble_mass_kernel_mex.cpp:46-127
 //   k_jacobian_coo   src/mex/assemble_jacobian_kernel_mex.cpp:61-152
+//   k_scalar_laplacian_coo  src/mex/assemble_scalar_laplacian_kernel_mex.cpp:41-104
+//   k_coupling_coo   src/mex/assemble_coupling_kernel_mex.cpp:44-131
+//   k_winding_coo    src/mex/assemble_winding_kernel_mex.cpp:48-125
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -131,6 +134,34 @@ __device__ __forceinline__ void stage_store6(double* stage, const double (&v)[6]
         out[e0 * 6 + i] = stage[el * 7 + k];
     }
     __syncthreads();
+}
+
+// Writes N values per lane through LDS into out[N*e0 .. N*(e0+count))
+// (stride N + 1: odd, against LDS bank conflicts).
+template <int N>
+__device__ __forceinline__ void stage_store(double* stage, const double (&v)[N],
+                                            double* __restrict__ out, int64_t e0, int count,
+                                            bool active) {
+    const int lane = threadIdx.x;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) stage[lane * (N + 1) + k] = v[k];
+    }
+    __syncthreads();
+    const int total = count * N;
+    for (int i = lane; i < total; i += kBlock) {
+        const int el = i / N, k = i - el * N;
+        out[e0 * N + i] = stage[el * (N + 1) + k];
+    }
+    __syncthreads();
+}
+
+// y = J^{-T} g for a reference row g (MexElemUtils multVecTrans), Ji row-major.
+__device__ __forceinline__ void jit_row(const double* Ji, double g0, double g1, double g2,
+                                        double* y) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) y[a] = Ji[0 + a] * g0 + Ji[3 + a] * g1 + Ji[6 + a] * g2;
 }
 
 // Fills I, J, V for a finished block K[r][c] (reference output loop:
@@ -398,6 +429,228 @@ k_jacobian_coo(const double* __restrict__ P, int64_t nn, const double* __restric
     }
 }
 
+// Reference gradient table of the P1 nodal functions, [4 x 3 x nq] at
+// i + 4 c + 12 q (lagrange_tet_p1.m:22-38), passed by value.
+struct GradArg {
+    double g[96];
+};
+
+// K = (sigma grad phi_i, grad phi_j): slot 16 e + 4 c + r = (dofs[r], dofs[c], K[r][c]).
+__global__ void __launch_bounds__(kBlock)
+k_scalar_laplacian_coo(const double* __restrict__ P, int64_t nn, const double* __restrict__ T,
+                       int64_t ne, const double* __restrict__ D, const double* __restrict__ C,
+                       QuadArg qa, GradArg ga, double* __restrict__ Io, double* __restrict__ Jo,
+                       double* __restrict__ Vo, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    __shared__ double stage[kBlock * 17];
+    const int64_t e0 = (int64_t)blockIdx.x * kBlock;
+    const int64_t e = e0 + threadIdx.x;
+    const int count = (int)((ne - e0) < kBlock ? (ne - e0) : kBlock);
+    const bool active = e < ne;
+    double K[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) K[i][j] = 0.0;
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    if (active) {
+        double p[12], J[9], Ji[9];
+        bool bad = false;
+        gather_nodes(P, nn, T, e, p, bad);
+        if (bad) atomicOr(err, 1);
+        const double detJ = jacobian_3d(p, J);
+        const double absDetJ = fabs(detJ);
+        inverse_3x3(J, detJ, Ji);
+        const double sigma = C[e];
+        if (fabs(sigma) > 1e-12) {
+            for (int q = 0; q < qa.nq; ++q) {
+                const double w = qa.qw[q] * absDetJ;
+                double G[3][4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    double y[3];
+                    jit_row(Ji, ga.g[i + 0 + 12 * q], ga.g[i + 4 + 12 * q], ga.g[i + 8 + 12 * q], y);
+                    G[0][i] = y[0];
+                    G[1][i] = y[1];
+                    G[2][i] = y[2];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const double dot =
+                            G[0][i] * G[0][j] + G[1][i] * G[1][j] + G[2][i] * G[2][j];
+                        K[i][j] += dot * sigma * w;
+                    }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[k] = (double)(long)D[k + e * 4];
+    }
+    double v[16];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[4 * c + r] = d[r];
+    stage_store<16>(stage, v, Io, e0, count, active);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[4 * c + r] = d[c];
+    stage_store<16>(stage, v, Jo, e0, count, active);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[4 * c + r] = K[r][c];
+    stage_store<16>(stage, v, Vo, e0, count, active);
+}
+
+// C = (sigma N_i, grad phi_j), 6 edge rows x 4 node columns:
+// slot 24 e + 6 c + r = (dofsE[r], dofsN[c], C[r][c] * s[r]).
+__global__ void __launch_bounds__(kBlock)
+k_coupling_coo(const double* __restrict__ P, int64_t nn, const double* __restrict__ T,
+               int64_t ne, const double* __restrict__ DE, const double* __restrict__ S,
+               const double* __restrict__ DN, const double* __restrict__ C, QuadArg qa,
+               GradArg ga, double* __restrict__ Io, double* __restrict__ Jo,
+               double* __restrict__ Vo, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    __shared__ double stage[kBlock * 25];
+    const int64_t e0 = (int64_t)blockIdx.x * kBlock;
+    const int64_t e = e0 + threadIdx.x;
+    const int count = (int)((ne - e0) < kBlock ? (ne - e0) : kBlock);
+    const bool active = e < ne;
+    double Cl[6][4];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Cl[i][j] = 0.0;
+    double re[6] = {0, 0, 0, 0, 0, 0}, sg[6] = {0, 0, 0, 0, 0, 0}, cn[4] = {0, 0, 0, 0};
+    if (active) {
+        double p[12], J[9], Ji[9];
+        bool bad = false;
+        gather_nodes(P, nn, T, e, p, bad);
+        if (bad) atomicOr(err, 1);
+        const double detJ = jacobian_3d(p, J);
+        const double absDetJ = fabs(detJ);
+        inverse_3x3(J, detJ, Ji);
+        const double sigma = C[e];
+        if (fabs(sigma) > 1e-12) {
+            for (int q = 0; q < qa.nq; ++q) {
+                const double w = qa.qw[q] * absDetJ;
+                double N[3][6], G[3][4];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double y[3];
+                    jit_row(Ji, qa.ref[i + 0 + 18 * q], qa.ref[i + 6 + 18 * q],
+                            qa.ref[i + 12 + 18 * q], y);
+                    N[0][i] = y[0];
+                    N[1][i] = y[1];
+                    N[2][i] = y[2];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    double y[3];
+                    jit_row(Ji, ga.g[i + 0 + 12 * q], ga.g[i + 4 + 12 * q], ga.g[i + 8 + 12 * q], y);
+                    G[0][i] = y[0];
+                    G[1][i] = y[1];
+                    G[2][i] = y[2];
+                }
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const double dot =
+                            N[0][i] * G[0][j] + N[1][i] * G[1][j] + N[2][i] * G[2][j];
+                        Cl[i][j] += dot * sigma * w;
+                    }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            re[k] = (double)(long)DE[k + e * 6];
+            sg[k] = S[k + e * 6];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cn[k] = (double)(long)DN[k + e * 4];
+    }
+    double v[24];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) v[6 * c + r] = re[r];
+    stage_store<24>(stage, v, Io, e0, count, active);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) v[6 * c + r] = cn[c];
+    stage_store<24>(stage, v, Jo, e0, count, active);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) v[6 * c + r] = Cl[r][c] * sg[r];
+    stage_store<24>(stage, v, Vo, e0, count, active);
+}
+
+// Winding source F_i = (Scale J_dir, N_i) on the elements of one coil
+// (tag == coil_id as ints); other elements give (0, 0) pairs.
+__global__ void __launch_bounds__(kBlock)
+k_winding_coo(const double* __restrict__ P, int64_t nn, const double* __restrict__ T,
+              int64_t ne, const double* __restrict__ D, const double* __restrict__ S,
+              const double* __restrict__ Tags, int coil, const double* __restrict__ Dir,
+              double u0, double u1, double u2, double scale, QuadArg qa,
+              double* __restrict__ Io, double* __restrict__ Vo, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    __shared__ double stage[kBlock * 7];
+    const int64_t e0 = (int64_t)blockIdx.x * kBlock;
+    const int64_t e = e0 + threadIdx.x;
+    const int count = (int)((ne - e0) < kBlock ? (ne - e0) : kBlock);
+    const bool active = e < ne;
+    double idx[6] = {0, 0, 0, 0, 0, 0}, val[6] = {0, 0, 0, 0, 0, 0};
+    if (active && (int)Tags[e] == coil) {
+        double p[12], J[9], Ji[9];
+        bool bad = false;
+        gather_nodes(P, nn, T, e, p, bad);
+        if (bad) atomicOr(err, 1);
+        const double detJ = jacobian_3d(p, J);
+        const double absDetJ = fabs(detJ);
+        inverse_3x3(J, detJ, Ji);
+        double jp[3];
+        if (Dir) {
+            jp[0] = Dir[0 + e * 3];
+            jp[1] = Dir[1 + e * 3];
+            jp[2] = Dir[2 + e * 3];
+        } else {
+            jp[0] = u0;
+            jp[1] = u1;
+            jp[2] = u2;
+        }
+        jp[0] *= scale;
+        jp[1] *= scale;
+        jp[2] *= scale;
+        double jt[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            jt[i] = Ji[3 * i + 0] * jp[0] + Ji[3 * i + 1] * jp[1] + Ji[3 * i + 2] * jp[2];
+        double F[6] = {0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < qa.nq; ++q) {
+            const double w = qa.qw[q] * absDetJ;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                const double v = jt[0] * qa.ref[i + 0 + 18 * q] + jt[1] * qa.ref[i + 6 + 18 * q] +
+                                 jt[2] * qa.ref[i + 12 + 18 * q];
+                F[i] += v * w;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            idx[i] = (double)(long)D[i + e * 6];
+            val[i] = F[i] * S[i + e * 6];
+        }
+    }
+    stage_store<6>(stage, idx, Io, e0, count, active);
+    stage_store<6>(stage, val, Vo, e0, count, active);
+}
+
 int check_quad(const mpk_quadrature* q) {
     MPK_REQUIRE(q != nullptr, MPK_ERR_ARGS, "quadrature table is NULL");
     MPK_REQUIRE(q->nq >= 1 && q->nq <= 8, MPK_ERR_ARGS, "nq=%d outside [1, 8]", q->nq);
@@ -509,6 +762,73 @@ int mpk_jacobian_coo(const double* P, int64_t nn, const double* T, int64_t ne,
             hipLaunchKernelGGL(k_jacobian_coo<false>, dim3(grid_for(ne, kBlock)), dim3(kBlock),
                                0, st, P, nn, T, ne, cell_dofs, signs, tags, sol_a, num_sol, qa,
                                mv, ntags, nbrk, ncoe, calc_j, I_out, J_out, V_out, R_idx, R_val, err);
+    });
+}
+
+int mpk_scalar_laplacian_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                             const double* node_dofs, const double* coeffs,
+                             const mpk_quadrature* quad, const double* grad_ref,
+                             double* I_out, double* J_out, double* V_out, void* stream) {
+    clear_error();
+    if (int rc = check_quad(quad)) return rc;
+    MPK_REQUIRE(ne >= 0 && nn >= 0, MPK_ERR_ARGS, "negative size");
+    if (ne == 0) return MPK_OK;
+    MPK_REQUIRE(P && T && node_dofs && coeffs && grad_ref && I_out && J_out && V_out,
+                MPK_ERR_ARGS, "NULL array argument");
+    MPK_REQUIRE(nn >= 1, MPK_ERR_INDEX, "elements given but no nodes");
+    const QuadArg qa = make_quad_arg(quad, false);
+    GradArg ga;
+    std::memset(&ga, 0, sizeof(ga));
+    for (int i = 0; i < 12 * quad->nq; ++i) ga.g[i] = grad_ref[i];
+    hipStream_t st = as_stream(stream);
+    return run_checked(st, [&](int* err) {
+        hipLaunchKernelGGL(k_scalar_laplacian_coo, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, st,
+                           P, nn, T, ne, node_dofs, coeffs, qa, ga, I_out, J_out, V_out, err);
+    });
+}
+
+int mpk_coupling_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                     const double* edge_dofs, const double* signs, const double* node_dofs,
+                     const double* coeffs, const mpk_quadrature* quad, const double* grad_ref,
+                     double* I_out, double* J_out, double* V_out, void* stream) {
+    clear_error();
+    if (int rc = check_quad(quad)) return rc;
+    MPK_REQUIRE(ne >= 0 && nn >= 0, MPK_ERR_ARGS, "negative size");
+    if (ne == 0) return MPK_OK;
+    MPK_REQUIRE(P && T && edge_dofs && signs && node_dofs && coeffs && grad_ref && I_out &&
+                    J_out && V_out,
+                MPK_ERR_ARGS, "NULL array argument");
+    MPK_REQUIRE(nn >= 1, MPK_ERR_INDEX, "elements given but no nodes");
+    const QuadArg qa = make_quad_arg(quad, false);
+    GradArg ga;
+    std::memset(&ga, 0, sizeof(ga));
+    for (int i = 0; i < 12 * quad->nq; ++i) ga.g[i] = grad_ref[i];
+    hipStream_t st = as_stream(stream);
+    return run_checked(st, [&](int* err) {
+        hipLaunchKernelGGL(k_coupling_coo, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, st, P, nn,
+                           T, ne, edge_dofs, signs, node_dofs, coeffs, qa, ga, I_out, J_out, V_out,
+                           err);
+    });
+}
+
+int mpk_winding_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                    const double* cell_dofs, const double* signs, const double* tags,
+                    double coil_id, const double* dir_field, const double* uniform_dir,
+                    double scale, const mpk_quadrature* quad, double* I_out, double* V_out,
+                    void* stream) {
+    clear_error();
+    if (int rc = check_quad(quad)) return rc;
+    MPK_REQUIRE(ne >= 0 && nn >= 0, MPK_ERR_ARGS, "negative size");
+    if (ne == 0) return MPK_OK;
+    MPK_REQUIRE(P && T && cell_dofs && signs && tags && uniform_dir && I_out && V_out,
+                MPK_ERR_ARGS, "NULL array argument");
+    MPK_REQUIRE(nn >= 1, MPK_ERR_INDEX, "elements given but no nodes");
+    const QuadArg qa = make_quad_arg(quad, false);
+    hipStream_t st = as_stream(stream);
+    return run_checked(st, [&](int* err) {
+        hipLaunchKernelGGL(k_winding_coo, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, st, P, nn,
+                           T, ne, cell_dofs, signs, tags, (int)coil_id, dir_field, uniform_dir[0],
+                           uniform_dir[1], uniform_dir[2], scale, qa, I_out, V_out, err);
     });
 }
 

@@ -10,6 +10,15 @@ identifier as the MEX files that ``Assembler.m`` calls:
 - ``assemble_jacobian_kernel_mex(P, T, CellDofs, Signs, Tags, SolA, Qw,
   CurlRef, linNu, isNon, maxBSq, breaks, coefs, starts, counts, coeStarts,
   calcJ)`` src/mex/assemble_jacobian_kernel_mex.cpp:7-153 (17 inputs)
+- ``assemble_source_kernel_mex(P, T, Dofs, Signs, Js, Qw, ValRef)``
+  src/mex/assemble_source_kernel_mex.cpp:16-120 (7 inputs)
+- ``assemble_scalar_laplacian_kernel_mex(P, T, Dofs, Coeffs, Qw, GradRef)``
+  src/mex/assemble_scalar_laplacian_kernel_mex.cpp:13-105 (6 inputs)
+- ``assemble_coupling_kernel_mex(P, T, DofsE, Signs, DofsN, Coeffs, Qw, NedRef,
+  GradRef)`` src/mex/assemble_coupling_kernel_mex.cpp:13-131 (9 inputs)
+- ``assemble_winding_kernel_mex(P, T, Dofs, Signs, RegionTags, CoilID,
+  DirField, UniformDir, Scale, Qw, RefVal)``
+  src/mex/assemble_winding_kernel_mex.cpp:13-126 (11 inputs, DirField may be [])
 - ``sparse_gpu(I, J, V, m, n)``: ``valid = I>0 & J>0; sparse(...)``
   (src/Assembler.m:37-38) on the GPU.
 
@@ -134,6 +143,76 @@ def assemble_source_kernel_mex(*args):
     Io, Vo = _out(6 * ne), _out(6 * ne)
     L.check(L.lib.mpk_source_coo(L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]),
                                  L.ptr(d[4]), C.byref(q), L.ptr(Io), L.ptr(Vo), L.stream_ptr()))
+    return tuple(t[: 6 * ne].cpu().numpy() for t in (Io, Vo))
+
+
+def _grad_table(Qw, GradRef) -> np.ndarray:
+    nq = _colmajor(Qw).size
+    g = _colmajor(GradRef)
+    if g.size < 12 * nq:
+        raise L.MpkError(L.MPK_ERR_ARGS, "GradRef smaller than 4 x 3 x numel(Qw)")
+    return np.ascontiguousarray(g[: 12 * nq])
+
+
+def assemble_scalar_laplacian_kernel_mex(*args):
+    """[I, J, V] = assemble_scalar_laplacian_kernel_mex(P, T, Dofs, Coeffs, Qw, GradRef)
+    (src/mex/assemble_scalar_laplacian_kernel_mex.cpp:13-105); Dofs [4 x Ne]."""
+    _arity(args, 6)
+    P, T, Dofs, Coeffs, Qw, GradRef = args
+    ne = _ncols(T)
+    nn = _colmajor(P).size // 3
+    q = L.make_quadrature(_colmajor(Qw), None, None)
+    g = _grad_table(Qw, GradRef)
+    d = [_dev(_colmajor(x)) for x in (P, T, Dofs, Coeffs)]
+    Io, Jo, Vo = _out(16 * ne), _out(16 * ne), _out(16 * ne)
+    L.check(L.lib.mpk_scalar_laplacian_coo(
+        L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]), C.byref(q),
+        g.ctypes.data_as(C.c_void_p), L.ptr(Io), L.ptr(Jo), L.ptr(Vo), L.stream_ptr()))
+    return tuple(t[: 16 * ne].cpu().numpy() for t in (Io, Jo, Vo))
+
+
+def assemble_coupling_kernel_mex(*args):
+    """[I, J, V] = assemble_coupling_kernel_mex(P, T, DofsE, Signs, DofsN, Coeffs, Qw,
+    NedRef, GradRef) (src/mex/assemble_coupling_kernel_mex.cpp:13-131)."""
+    _arity(args, 9)
+    P, T, DofsE, Signs, DofsN, Coeffs, Qw, NedRef, GradRef = args
+    ne = _ncols(T)
+    nn = _colmajor(P).size // 3
+    q = _quad(Qw, NedRef, False)
+    g = _grad_table(Qw, GradRef)
+    d = [_dev(_colmajor(x)) for x in (P, T, DofsE, Signs, DofsN, Coeffs)]
+    Io, Jo, Vo = _out(24 * ne), _out(24 * ne), _out(24 * ne)
+    L.check(L.lib.mpk_coupling_coo(
+        L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]), L.ptr(d[4]), L.ptr(d[5]),
+        C.byref(q), g.ctypes.data_as(C.c_void_p), L.ptr(Io), L.ptr(Jo), L.ptr(Vo),
+        L.stream_ptr()))
+    return tuple(t[: 24 * ne].cpu().numpy() for t in (Io, Jo, Vo))
+
+
+def assemble_winding_kernel_mex(*args):
+    """[I, V] = assemble_winding_kernel_mex(P, T, Dofs, Signs, RegionTags, CoilID,
+    DirField, UniformDir, Scale, Qw, RefVal)
+    (src/mex/assemble_winding_kernel_mex.cpp:13-126); DirField [3 x Ne] or []."""
+    _arity(args, 11)
+    P, T, Dofs, Signs, Tags, CoilID, DirField, UniDir, Scale, Qw, RefVal = args
+    ne = _ncols(T)
+    nn = _colmajor(P).size // 3
+    q = _quad(Qw, RefVal, False)
+    uni = _colmajor(UniDir)
+    if uni.size < 3:
+        raise L.MpkError(L.MPK_ERR_ARGS, "UniformDir must hold 3 values")
+    d = [_dev(_colmajor(x)) for x in (P, T, Dofs, Signs, Tags)]
+    dirf = None
+    if np.size(DirField):
+        df = np.asarray(DirField, dtype=np.float64)
+        if df.ndim != 2 or df.shape[0] != 3 or df.shape[1] != ne:
+            raise L.MpkError(L.MPK_ERR_ARGS, "DirField must be [3 x NumElems] or []")
+        dirf = _dev(_colmajor(df))
+    Io, Vo = _out(6 * ne), _out(6 * ne)
+    L.check(L.lib.mpk_winding_coo(
+        L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]), L.ptr(d[4]),
+        float(np.asarray(CoilID).ravel()[0]), L.ptr(dirf), uni[:3].ctypes.data_as(C.c_void_p),
+        float(np.asarray(Scale).ravel()[0]), C.byref(q), L.ptr(Io), L.ptr(Vo), L.stream_ptr()))
     return tuple(t[: 6 * ne].cpu().numpy() for t in (Io, Vo))
 
 

@@ -22,6 +22,9 @@
  *   orc_eval_nu          src/mex/MexMaterialUtils.hpp:24-87 (evaluate_nu_derivative)
  *   orc_jacobian_kernel  src/mex/assemble_jacobian_kernel_mex.cpp:7-153
  *   orc_source_kernel    src/mex/assemble_source_kernel_mex.cpp:16-120
+ *   orc_scalar_laplacian_kernel  src/mex/assemble_scalar_laplacian_kernel_mex.cpp:13-105
+ *   orc_coupling_kernel  src/mex/assemble_coupling_kernel_mex.cpp:13-131
+ *   orc_winding_kernel   src/mex/assemble_winding_kernel_mex.cpp:13-126
  *   orc_element_b        src/post/PostProcessor.m:64-126 (computeElementB)
  *   orc_sparse           src/Assembler.m:37-38  (valid = I>0 & J>0; MATLAB sparse():
  *                        duplicates summed, CSC with sorted rows, zeros dropped;
@@ -583,5 +586,154 @@ void orc_element_b(const double* P, const double* T, const double* Dofs, const d
         B[3 * e + 0] = b[0];
         B[3 * e + 1] = b[1];
         B[3 * e + 2] = b[2];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Nodal kernels of the A-V formulation and the winding source         */
+/* ------------------------------------------------------------------ */
+/* J^{-T} g for a reference row g (MexElemUtils multVecTrans). */
+static inline void orc_jit(const double* Ji, const double* g, double* y) {
+    for (int a = 0; a < 3; a++) y[a] = Ji[0 + a] * g[0] + Ji[3 + a] * g[1] + Ji[6 + a] * g[2];
+}
+
+/*
+ * K = (sigma grad phi_i, grad phi_j) over P1 nodal functions.  Dofs: 4 x ne
+ * (1-based node DOFs, as doubles); GradRef [4 x 3 x nq] at i + 4c + 12q.
+ * Output slot 16 e + 4 c + r = (dofs[r], dofs[c], K[r][c]); elements with
+ * |sigma| <= 1e-12 give a zero block (their indices are still written).
+ */
+void orc_scalar_laplacian_kernel(const double* P, const double* T, const double* Dofs,
+                                 const double* Coeffs, const double* Qw, int64_t nq,
+                                 const double* GradRef, int64_t ne, double* I, double* Jo,
+                                 double* V) {
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < ne; e++) {
+        double p[12], J[9], Ji[9];
+        orc_gather(P, T, e, p);
+        double detJ = orc_jacobian_3d(p, J);
+        double absDetJ = fabs(detJ);
+        orc_inverse_3x3(J, detJ, Ji);
+        double K[4][4];
+        memset(K, 0, sizeof(K));
+        double sigma = Coeffs[e];
+        if (fabs(sigma) > 1e-12) {
+            for (int64_t q = 0; q < nq; q++) {
+                double w = Qw[q] * absDetJ;
+                double G[3][4];
+                for (int i = 0; i < 4; i++) {
+                    double g[3] = {GradRef[i + 0 + 12 * q], GradRef[i + 4 + 12 * q],
+                                   GradRef[i + 8 + 12 * q]}, y[3];
+                    orc_jit(Ji, g, y);
+                    for (int a = 0; a < 3; a++) G[a][i] = y[a];
+                }
+                for (int i = 0; i < 4; i++)
+                    for (int j = 0; j < 4; j++) {
+                        double dot = G[0][i] * G[0][j] + G[1][i] * G[1][j] + G[2][i] * G[2][j];
+                        K[i][j] += dot * sigma * w;
+                    }
+            }
+        }
+        for (int c = 0; c < 4; c++)
+            for (int r = 0; r < 4; r++) {
+                long k = 16 * e + 4 * c + r;
+                I[k] = (double)(long)Dofs[r + 4 * e];
+                Jo[k] = (double)(long)Dofs[c + 4 * e];
+                V[k] = K[r][c];
+            }
+    }
+}
+
+/*
+ * C = (sigma N_i, grad phi_j): 6 edge rows x 4 node columns.  DofsE/Signs 6 x ne,
+ * DofsN 4 x ne; NedRef [6 x 3 x nq] at i + 6c + 18q, GradRef [4 x 3 x nq].
+ * Output slot 24 e + 6 c + r = (dofsE[r], dofsN[c], C[r][c] * s[r]).
+ */
+void orc_coupling_kernel(const double* P, const double* T, const double* DofsE,
+                         const double* Signs, const double* DofsN, const double* Coeffs,
+                         const double* Qw, int64_t nq, const double* NedRef,
+                         const double* GradRef, int64_t ne, double* I, double* Jo, double* V) {
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < ne; e++) {
+        double p[12], J[9], Ji[9];
+        orc_gather(P, T, e, p);
+        double detJ = orc_jacobian_3d(p, J);
+        double absDetJ = fabs(detJ);
+        orc_inverse_3x3(J, detJ, Ji);
+        double Cl[6][4];
+        memset(Cl, 0, sizeof(Cl));
+        double sigma = Coeffs[e];
+        if (fabs(sigma) > 1e-12) {
+            for (int64_t q = 0; q < nq; q++) {
+                double w = Qw[q] * absDetJ;
+                double N[3][6], G[3][4];
+                for (int i = 0; i < 6; i++) {
+                    double g[3] = {NedRef[i + 0 + 18 * q], NedRef[i + 6 + 18 * q],
+                                   NedRef[i + 12 + 18 * q]}, y[3];
+                    orc_jit(Ji, g, y);
+                    for (int a = 0; a < 3; a++) N[a][i] = y[a];
+                }
+                for (int i = 0; i < 4; i++) {
+                    double g[3] = {GradRef[i + 0 + 12 * q], GradRef[i + 4 + 12 * q],
+                                   GradRef[i + 8 + 12 * q]}, y[3];
+                    orc_jit(Ji, g, y);
+                    for (int a = 0; a < 3; a++) G[a][i] = y[a];
+                }
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 4; j++) {
+                        double dot = N[0][i] * G[0][j] + N[1][i] * G[1][j] + N[2][i] * G[2][j];
+                        Cl[i][j] += dot * sigma * w;
+                    }
+            }
+        }
+        for (int c = 0; c < 4; c++)
+            for (int r = 0; r < 6; r++) {
+                long k = 24 * e + 6 * c + r;
+                I[k] = (double)(long)DofsE[r + 6 * e];
+                Jo[k] = (double)(long)DofsN[c + 4 * e];
+                V[k] = Cl[r][c] * Signs[r + 6 * e];
+            }
+    }
+}
+
+/*
+ * Winding source F_i = (Scale J_dir, N_i) over the elements whose tag equals
+ * coil_id (as ints); other elements give (0, 0) pairs.  J_dir per element
+ * from DirField [3 x ne] when given, else UniDir.  Output slot 6 e + i =
+ * (dofs[i], F_i s[i]).
+ */
+void orc_winding_kernel(const double* P, const double* T, const double* Dofs,
+                        const double* Signs, const double* Tags, double coil_id,
+                        const double* DirField, const double* UniDir, double scale,
+                        const double* Qw, int64_t nq, const double* RefVal, int64_t ne,
+                        double* I, double* V) {
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < ne; e++) {
+        if ((int)Tags[e] != (int)coil_id) {
+            for (int i = 0; i < 6; i++) I[6 * e + i] = V[6 * e + i] = 0.0;
+            continue;
+        }
+        double p[12], J[9], Ji[9];
+        orc_gather(P, T, e, p);
+        double detJ = orc_jacobian_3d(p, J);
+        double absDetJ = fabs(detJ);
+        orc_inverse_3x3(J, detJ, Ji);
+        double jp[3];
+        for (int a = 0; a < 3; a++) jp[a] = (DirField ? DirField[a + 3 * e] : UniDir[a]) * scale;
+        double jt[3];
+        for (int i = 0; i < 3; i++) jt[i] = Ji[3 * i + 0] * jp[0] + Ji[3 * i + 1] * jp[1] + Ji[3 * i + 2] * jp[2];
+        double F[6] = {0, 0, 0, 0, 0, 0};
+        for (int64_t q = 0; q < nq; q++) {
+            double w = Qw[q] * absDetJ;
+            for (int i = 0; i < 6; i++) {
+                double val = jt[0] * RefVal[i + 0 + 18 * q] + jt[1] * RefVal[i + 6 + 18 * q] +
+                             jt[2] * RefVal[i + 12 + 18 * q];
+                F[i] += val * w;
+            }
+        }
+        for (int i = 0; i < 6; i++) {
+            I[6 * e + i] = (double)(long)Dofs[i + 6 * e];
+            V[6 * e + i] = F[i] * Signs[i + 6 * e];
+        }
     }
 }

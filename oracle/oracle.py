@@ -136,6 +136,45 @@ def source(P, T, Dofs, Signs, Js, Qw, ValRef):
     return I, V
 
 
+def scalar_laplacian(P, T, Dofs, Coeffs, Qw, GradRef):
+    """[I, J, V] = assemble_scalar_laplacian_kernel_mex(P, T, Dofs, Coeffs, Qw, GradRef);
+    Dofs (Ne, 4) node DOFs, GradRef flat [4 x 3 x Nq]."""
+    Pp = _d(P); Tp = _d(T); Dp = _d(Dofs); Cp = _d(Coeffs); Qp = _d(Qw); Gp = _d(GradRef)
+    ne = Tp[0].size // 4
+    I = np.zeros(16 * ne); J = np.zeros(16 * ne); V = np.zeros(16 * ne)
+    _lib.orc_scalar_laplacian_kernel(Pp[1], Tp[1], Dp[1], Cp[1], Qp[1], _i64(Qp[0].size), Gp[1],
+                                     _i64(ne), I.ctypes.data_as(_dp), J.ctypes.data_as(_dp),
+                                     V.ctypes.data_as(_dp))
+    return I, J, V
+
+
+def coupling(P, T, DofsE, Signs, DofsN, Coeffs, Qw, NedRef, GradRef):
+    """[I, J, V] = assemble_coupling_kernel_mex(P, T, DofsE, Signs, DofsN, Coeffs, Qw,
+    NedRef, GradRef): 24 triplets per element."""
+    Pp = _d(P); Tp = _d(T); Ep = _d(DofsE); Sp = _d(Signs); Np = _d(DofsN); Cp = _d(Coeffs)
+    Qp = _d(Qw); Rp = _d(NedRef); Gp = _d(GradRef)
+    ne = Tp[0].size // 4
+    I = np.zeros(24 * ne); J = np.zeros(24 * ne); V = np.zeros(24 * ne)
+    _lib.orc_coupling_kernel(Pp[1], Tp[1], Ep[1], Sp[1], Np[1], Cp[1], Qp[1], _i64(Qp[0].size),
+                             Rp[1], Gp[1], _i64(ne), I.ctypes.data_as(_dp),
+                             J.ctypes.data_as(_dp), V.ctypes.data_as(_dp))
+    return I, J, V
+
+
+def winding(P, T, Dofs, Signs, Tags, CoilID, DirField, UniDir, Scale, Qw, RefVal):
+    """[I, V] = assemble_winding_kernel_mex(P, T, Dofs, Signs, RegionTags, CoilID,
+    DirField (Ne, 3) or None, UniformDir (3,), Scale, Qw, RefVal)."""
+    Pp = _d(P); Tp = _d(T); Dp = _d(Dofs); Sp = _d(Signs); Gp = _d(Tags); Up = _d(UniDir)
+    Qp = _d(Qw); Rp = _d(RefVal)
+    Fp = _d(DirField) if DirField is not None and np.size(DirField) else (None, None)
+    ne = Tp[0].size // 4
+    I = np.zeros(6 * ne); V = np.zeros(6 * ne)
+    _lib.orc_winding_kernel(Pp[1], Tp[1], Dp[1], Sp[1], Gp[1], C.c_double(float(CoilID)),
+                            Fp[1], Up[1], C.c_double(float(Scale)), Qp[1], _i64(Qp[0].size),
+                            Rp[1], _i64(ne), I.ctypes.data_as(_dp), V.ctypes.data_as(_dp))
+    return I, V
+
+
 def element_b(P, T, Dofs, Signs, sol, CurlRef):
     """PostProcessor.computeElementB for one solution column: B (Ne, 3)."""
     Pp = _d(P); Tp = _d(T); Dp = _d(Dofs); Sp = _d(Signs); sp = _d(sol); Cp = _d(CurlRef)

@@ -114,6 +114,53 @@ def test_coo_kernels_bit_exact_mesh(gpu, ne_cubes):
             assert np.array_equal(np.asarray(g).view(np.uint64), r.view(np.uint64))
 
 
+def _nodal_tables():
+    from magpackage_amd.fem_tables import get_quadrature_data, lagrange_tet_p1, matlab_flat
+    q_pts, qw = get_quadrature_data("tet", 2)
+    _, lg = lagrange_tet_p1(q_pts)
+    _, val, _ = mass_tables(2)
+    return qw, matlab_flat(lg), val
+
+
+def test_nodal_and_winding_coo_bit_exact(gpu):
+    """assemble_scalar_laplacian / coupling / winding_kernel_mex drop-ins against
+    the oracle, bit for bit (a zero-sigma region, inactive node DOFs, both
+    DirField forms, elements outside the coil)."""
+    from magpackage_amd import mex
+    mesh = config1_mesh(6, seed=9)
+    mi = O.mesh_inputs(mesh)
+    ne = mesh.num_elems
+    rng = np.random.default_rng(40)
+    qw, G, val = _nodal_tables()
+    T = mi["T"]
+    # node DOFs: a permutation of the nodes, node 1 inactive (0)
+    perm = rng.permutation(mesh.num_nodes) + 1.0
+    perm[0] = 0.0
+    DN = perm[(T - 1).astype(np.int64)]
+    sig = np.where(mesh.RegionTags == 2, 0.0, rng.uniform(1e6, 5e7, ne))
+    I, J, V = mex.assemble_scalar_laplacian_kernel_mex(mi["P"].T, T.T, DN.T, sig, qw, G)
+    Io, Jo, Vo = O.scalar_laplacian(mi["P"], T, DN, sig, qw, G)
+    assert np.array_equal(I, Io) and np.array_equal(J, Jo)
+    assert np.array_equal(V.view(np.uint64), Vo.view(np.uint64))
+    I, J, V = mex.assemble_coupling_kernel_mex(mi["P"].T, T.T, mi["Dofs"].T, mi["Signs"].T, DN.T,
+                                               sig, qw, val, G)
+    Io, Jo, Vo = O.coupling(mi["P"], T, mi["Dofs"], mi["Signs"], DN, sig, qw, val, G)
+    assert np.array_equal(I, Io) and np.array_equal(J, Jo)
+    assert np.array_equal(V.view(np.uint64), Vo.view(np.uint64))
+    tags = mesh.RegionTags.astype(np.float64)
+    dirf = rng.normal(size=(ne, 3))
+    for field in (dirf, None):
+        I, V = mex.assemble_winding_kernel_mex(mi["P"].T, T.T, mi["Dofs"].T, mi["Signs"].T, tags,
+                                               2, field.T if field is not None else np.zeros(0),
+                                               np.array([0.0, 0.0, 1.0]), 3.5e6, qw, val)
+        Io, Vo = O.winding(mi["P"], T, mi["Dofs"], mi["Signs"], tags, 2, field,
+                           np.array([0.0, 0.0, 1.0]), 3.5e6, qw, val)
+        assert np.array_equal(I, Io)
+        assert np.array_equal(V.view(np.uint64), Vo.view(np.uint64))
+    with pytest.raises(Exception, match="Expected 6 inputs"):
+        mex.assemble_scalar_laplacian_kernel_mex(mi["P"].T, T.T, DN.T, sig, qw)
+
+
 def test_coo_index_errors(gpu):
     from magpackage_amd import mex
     from magpackage_amd._lib import MpkError

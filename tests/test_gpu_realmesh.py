@@ -112,6 +112,38 @@ def test_real_coo_mex_bit_exact(real):
         assert np.array_equal(np.asarray(g).view(np.uint64), r.view(np.uint64))
 
 
+def test_real_nodal_and_winding_coo(real):
+    """A-V coupling, scalar Laplacian and the winding source on the real mesh,
+    bit for bit against the oracle; the winding runs per coil region (the
+    E-core's coil parts are regions 3-10, TEAM7's coil regions 3-6)."""
+    from magpackage_amd import mex
+    from magpackage_amd.fem_tables import get_quadrature_data, lagrange_tet_p1, matlab_flat
+    mi, raw = real["mi"], real["raw"]
+    q_pts, qw = get_quadrature_data("tet", 2)
+    G = matlab_flat(lagrange_tet_p1(q_pts)[1])
+    _, val, _ = mass_tables(2)
+    T = mi["T"]
+    DN = T.copy()  # full-domain P1: node DOF = node id
+    sig = real["sigma"]
+    I, J, V = mex.assemble_scalar_laplacian_kernel_mex(mi["P"].T, T.T, DN.T, sig, qw, G)
+    Io, Jo, Vo = O.scalar_laplacian(mi["P"], T, DN, sig, qw, G)
+    assert np.array_equal(I, Io) and np.array_equal(J, Jo)
+    assert np.array_equal(V.view(np.uint64), Vo.view(np.uint64))
+    I, J, V = mex.assemble_coupling_kernel_mex(mi["P"].T, T.T, mi["Dofs"].T, mi["Signs"].T, DN.T,
+                                               sig, qw, val, G)
+    Io, Jo, Vo = O.coupling(mi["P"], T, mi["Dofs"], mi["Signs"], DN, sig, qw, val, G)
+    assert np.array_equal(V.view(np.uint64), Vo.view(np.uint64))
+    tags = raw.RegionTags.astype(np.float64)
+    for coil in (3, 5):
+        I, V = mex.assemble_winding_kernel_mex(mi["P"].T, T.T, mi["Dofs"].T, mi["Signs"].T, tags,
+                                               coil, np.zeros(0), np.array([1.0, 0.0, 0.0]),
+                                               -120.0 / 2.5e-4, qw, val)
+        Io, Vo = O.winding(mi["P"], T, mi["Dofs"], mi["Signs"], tags, coil, None,
+                           np.array([1.0, 0.0, 0.0]), -120.0 / 2.5e-4, qw, val)
+        assert np.array_equal(I, Io) and (I > 0).any()
+        assert np.array_equal(V.view(np.uint64), Vo.view(np.uint64))
+
+
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
 def test_real_stiffness_mass(real, prec, driver):
     if prec == "fp64_exact" and driver != "ring":

@@ -387,3 +387,94 @@ def test_element_b_against_whitney_curls():
         c, _ = whitney_curls(P[T[e] - 1])
         a = np.where(D[e] > 0, sol[np.maximum(D[e].astype(int), 1) - 1], 0.0) * S[e]
         assert rel_err(B[e], a @ c) < 1e-12
+
+
+def _grad_ref_table():
+    from magpackage_amd.fem_tables import get_quadrature_data, lagrange_tet_p1, matlab_flat
+    q_pts, qw = get_quadrature_data("tet", 2)
+    _, lg = lagrange_tet_p1(q_pts)
+    return qw, matlab_flat(lg)
+
+
+def test_scalar_laplacian_closed_form_random_tets():
+    """K_ij = sigma |T| grad l_i . grad l_j (P1 gradients are constant), zero
+    blocks for |sigma| <= 1e-12, node DOFs passed through (0 = inactive)."""
+    P, T = random_tets(120, seed=31)
+    ne = T.shape[0]
+    rng = np.random.default_rng(32)
+    D = rng.integers(0, 50, size=(ne, 4)).astype(np.float64)
+    sig = rng.uniform(1e6, 5e7, ne)
+    sig[::5] = 1e-13
+    qw, G = _grad_ref_table()
+    I, J, V = O.scalar_laplacian(P, T.astype(np.float64), D, sig, qw, G)
+    worst = 0.0
+    for e in range(ne):
+        g, det = grads(P[T[e] - 1])
+        K = sig[e] * abs(det) / 6.0 * (g @ g.T)
+        blk = slice(16 * e, 16 * e + 16)
+        assert np.array_equal(I[blk], np.tile(D[e], 4))
+        assert np.array_equal(J[blk], np.repeat(D[e], 4))
+        if sig[e] <= 1e-12:
+            assert not V[blk].any()
+            continue
+        worst = max(worst, rel_err(V[blk], K.T.ravel()))
+    assert worst < 1e-12
+
+
+def test_coupling_closed_form_random_tets():
+    """C_ij = s_i sigma (grad l_j) . int W_i = s_i sigma (|T|/4)(grad l_b - grad l_a) .
+    grad l_j: the 4-point rule integrates the linear Whitney functions exactly."""
+    P, T = random_tets(120, seed=33)
+    ne = T.shape[0]
+    rng = np.random.default_rng(34)
+    S = rng.choice([-1.0, 1.0], size=(ne, 6))
+    DE = np.arange(1, 6 * ne + 1, dtype=np.float64).reshape(ne, 6)
+    DN = rng.integers(1, 40, size=(ne, 4)).astype(np.float64)
+    sig = rng.uniform(1e6, 5e7, ne)
+    sig[::6] = 0.0
+    qw, G = _grad_ref_table()
+    _, val, _ = mass_tables(2)
+    I, J, V = O.coupling(P, T.astype(np.float64), DE, S, DN, sig, qw, val, G)
+    worst = 0.0
+    for e in range(ne):
+        g, det = grads(P[T[e] - 1])
+        vol = abs(det) / 6.0
+        C = np.array([[S[e, i] * sig[e] * vol / 4.0 * (g[b] - g[a]) @ g[j] for j in range(4)]
+                      for i, (a, b) in enumerate(EDGE_NODES)])
+        blk = slice(24 * e, 24 * e + 24)
+        assert np.array_equal(I[blk], np.tile(DE[e], 4))
+        assert np.array_equal(J[blk], np.repeat(DN[e], 6))
+        if sig[e] == 0.0:
+            assert not V[blk].any()
+            continue
+        worst = max(worst, rel_err(V[blk], C.T.ravel()))
+    assert worst < 1e-12
+
+
+def test_winding_closed_form_random_tets():
+    """F_i = s_i Scale J_dir . (|T|/4)(grad l_b - grad l_a) on the coil's
+    elements, (0, 0) pairs elsewhere; DirField per element or UniformDir."""
+    P, T = random_tets(90, seed=35)
+    ne = T.shape[0]
+    rng = np.random.default_rng(36)
+    S = rng.choice([-1.0, 1.0], size=(ne, 6))
+    D = np.arange(1, 6 * ne + 1, dtype=np.float64).reshape(ne, 6)
+    tags = rng.choice([1.0, 5.0, 7.0], size=ne)
+    dirf = rng.normal(size=(ne, 3))
+    uni = np.array([0.3, -0.5, 0.8])
+    qw, val, _ = mass_tables(2)
+    for field in (dirf, None):
+        I, V = O.winding(P, T.astype(np.float64), D, S, tags, 5, field, uni, 250.0, qw, val)
+        worst = 0.0
+        for e in range(ne):
+            blk = slice(6 * e, 6 * e + 6)
+            if tags[e] != 5.0:
+                assert not I[blk].any() and not V[blk].any()
+                continue
+            g, det = grads(P[T[e] - 1])
+            jd = 250.0 * (dirf[e] if field is not None else uni)
+            ref = np.array([S[e, i] * jd @ (g[b] - g[a]) * abs(det) / 24.0
+                            for i, (a, b) in enumerate(EDGE_NODES)])
+            assert np.array_equal(I[blk], D[e])
+            worst = max(worst, rel_err(V[blk], ref))
+        assert worst < 1e-12
