@@ -155,6 +155,26 @@ int mpk_winding_coo(const double* P, int64_t nn, const double* T, int64_t ne,
                     double scale, const mpk_quadrature* quad, double* I_out, double* V_out,
                     void* stream);
 
+/* Replaces assemble_hbfem_kernel_mex (src/mex/assemble_hbfem_kernel_mex.cpp:25-260):
+ * harmonic-balance residual and DC-reluctivity block Jacobian.  sol_r / sol_i
+ * [num_dofs x num_harm] (sol_i may be NULL), dmat_r / dmat_i [num_time x num_harm]
+ * (AFT D matrix; dmat_i may be NULL), pmat_r / pmat_i [num_harm x num_time] (AFT P
+ * matrix; pmat_i may be NULL), column-major; scalings [num_harm] (Jacobian block
+ * factors).  The material table needs coef_start (the MEX leaves it
+ * uninitialised).  Outputs, in the MEX's slots: R rows / cols / real / imag at
+ * 6*H*e + 6*h + k; with request_j, 36*H*e triplets (dofs + h*num_dofs) at
+ * 36*H*e + 36*h + 6*c + r.  A DOF of 0 reads sol at h*num_dofs - 1 for h >= 1 (the
+ * MEX's index arithmetic) and 0 for h = 0.  num_harm <= 64.  Device arrays except
+ * quad and mat; synchronises `stream` for the index checks. */
+int mpk_hbfem_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                  const double* cell_dofs, const double* signs, const double* tags,
+                  const double* sol_r, const double* sol_i, int64_t num_dofs, int32_t num_harm,
+                  const double* dmat_r, const double* dmat_i, int32_t num_time,
+                  const double* pmat_r, const double* pmat_i, const mpk_quadrature* quad,
+                  const mpk_material* mat, int request_j, const double* scalings,
+                  double* I_out, double* J_out, double* V_out, double* R_rows, double* R_cols,
+                  double* R_val_r, double* R_val_i, void* stream);
+
 /* PostProcessor.computeElementB (src/post/PostProcessor.m:64-126) for one
  * solution column: B_out [3 x ne] device, B_e = curl A at the element.  T
  * [ne x 4] and cell_dofs [ne x 6] device int32, 1-based (0 = inactive DOF);

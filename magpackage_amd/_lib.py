@@ -87,6 +87,9 @@ _SIGS = {
                                   _vp], _i32),
     "mpk_coupling_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _pq, _vp, _vp, _vp, _vp,
                           _vp], _i32),
+    "mpk_hbfem_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _vp, _vp,
+                       _i32, _vp, _vp, _pq, _pm, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                       _vp], _i32),
     "mpk_winding_coo": ([_vp, _i64, _vp, _i64, _vp, _vp, _vp, C.c_double, _vp, _vp, C.c_double,
                          _pq, _vp, _vp, _vp], _i32),
     "mpk_element_b": ([_vp, _i64, _vp, _vp, _i64, _vp, _i64, _pq, _vp, _vp], _i32),

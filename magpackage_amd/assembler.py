@@ -453,6 +453,77 @@ class Assembler:
         del keep
         return {k: (v[: plan.n] if k == "R" else v[: plan.nnz]) for k, v in out.items()}
 
+    def assembleHBFEM(self, space, SolHarmonics, aft, matLibData, requestJacobian: bool = True):
+        """The harmonic-balance assembly HBFEMSolver calls (src/solver/HBFEMSolver.m:78,96;
+        the method is absent from the reference's Assembler, whose MATLAB kernel
+        src/kernels/assemble_hbfem_kernel.m:1-240 defines it) on the engine's HBFEM
+        kernel.  SolHarmonics [NumGlobalDofs x H] complex (host or device);
+        aft: hbfem.AFT.  Returns (J_tri, R): J_tri a dict of device tensors I, J, V
+        (1-based, the triplets with both indices > 0; None without
+        requestJacobian) of the block-diagonal DC-reluctivity Jacobian, R the
+        [NumGlobalDofs x H] complex128 residual (duplicates summed in triplet
+        order)."""
+        dev = _dev()
+        key = space.toString()
+        dofs = self.DofHandler.DofMaps[key]
+        nd = int(self.DofHandler.NumGlobalDofs)
+        sol = torch.as_tensor(SolHarmonics).to(dev)
+        H = int(aft.NumHarmonics)
+        if sol.dim() == 1:
+            sol = sol[:, None]
+        if sol.shape[0] != nd or sol.shape[1] != H:
+            raise L.MpkError(L.MPK_ERR_ARGS, f"SolHarmonics must be [{nd} x {H}]")
+        solc = sol.to(torch.complex128)
+        sr = solc.real.t().contiguous()  # column-major [nd x H]
+        si = solc.imag.t().contiguous()
+        M = self.Mesh
+        self.Mesh.generateEdges()
+        ne = M.NumElements
+        Td = M.T_dev.to(torch.float64)
+        Dd = dofs.reshape(-1).to(torch.float64)
+        Sd = M.T2E_Sign.reshape(-1).to(torch.float64)
+        tags = M.Tags_dev.to(torch.float64)
+        cm = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(dev).reshape(-1)
+        dr, di = cm(aft.D_matrix.real), cm(aft.D_matrix.imag)
+        pr, pi_ = cm(aft.P_matrix.real), cm(aft.P_matrix.imag)
+        sc = torch.from_numpy(aft.jacobian_scalings()).to(dev)
+        mat, keep = self.material_tables(matLibData)
+        q = self.stiffness_quadrature()
+        nk = 36 * H * ne if requestJacobian else 0
+        out = lambda n: torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+        Io, Jo, Vo = out(nk), out(nk), out(nk)
+        Rr, Rc, Rvr, Rvi = (out(6 * H * ne) for _ in range(4))
+        L.check(L.lib.mpk_hbfem_coo(
+            L.ptr(M.P_dev), M.NumNodes, L.ptr(Td), ne, L.ptr(Dd), L.ptr(Sd), L.ptr(tags),
+            L.ptr(sr), L.ptr(si), nd, H, L.ptr(dr), L.ptr(di), int(aft.NumTimeSteps), L.ptr(pr),
+            L.ptr(pi_), C.byref(q), C.byref(mat), int(bool(requestJacobian)), L.ptr(sc),
+            L.ptr(Io), L.ptr(Jo), L.ptr(Vo), L.ptr(Rr), L.ptr(Rc), L.ptr(Rvr), L.ptr(Rvi),
+            L.stream_ptr()))
+        del keep
+        n6 = 6 * H * ne
+        R = torch.zeros((nd, H), dtype=torch.complex128, device=dev)
+        for part, vals in ((0, Rvr[:n6]), (1, Rvi[:n6])):
+            colptr = torch.empty(H + 1, dtype=torch.int64, device=dev)
+            rowind = torch.empty(max(n6, 1), dtype=torch.int32, device=dev)
+            v = torch.empty(max(n6, 1), dtype=torch.float64, device=dev)
+            nnz = C.c_int64(0)
+            L.check(L.lib.mpk_sparse_coo_to_csc(L.ptr(Rr), L.ptr(Rc), L.ptr(vals), n6, nd, H, 0,
+                                                L.ptr(colptr), L.ptr(rowind), L.ptr(v), n6,
+                                                C.byref(nnz), L.stream_ptr()))
+            k = int(nnz.value)
+            cols = torch.repeat_interleave(torch.arange(H, device=dev), colptr.diff())
+            dense = torch.zeros((nd, H), dtype=torch.float64, device=dev)
+            dense[rowind[:k].long(), cols[:k]] = v[:k]
+            if part == 0:
+                R.real.copy_(dense)
+            else:
+                R.imag.copy_(dense)
+        J_tri = None
+        if requestJacobian:
+            ok = (Io[:nk] > 0) & (Jo[:nk] > 0)
+            J_tri = {"I": Io[:nk][ok], "J": Jo[:nk][ok], "V": Vo[:nk][ok]}
+        return J_tri, R
+
     def source_quadrature(self) -> L.Quadrature:
         return L.standard_quadrature(max(self.Config.DefaultQuadratureOrder, 2))
 

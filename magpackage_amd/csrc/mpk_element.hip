@@ -13,6 +13,7 @@
 //   k_scalar_laplacian_coo  src/mex/assemble_scalar_laplacian_kernel_mex.cpp:41-104
 //   k_coupling_coo   src/mex/assemble_coupling_kernel_mex.cpp:44-131
 //   k_winding_coo    src/mex/assemble_winding_kernel_mex.cpp:48-125
+//   k_hbfem_coo      src/mex/assemble_hbfem_kernel_mex.cpp:93-259
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -651,6 +652,155 @@ k_winding_coo(const double* __restrict__ P, int64_t nn, const double* __restrict
     stage_store<6>(stage, val, Vo, e0, count, active);
 }
 
+constexpr int kHbMaxHarm = 64;  // 24 * 64 doubles per lane: one lane per 12 KB block
+
+// Harmonic-balance residual and DC-reluctivity block Jacobian, one element
+// per lane.  The lane's phasors A (real, imaginary) and residual accumulators
+// (real, imaginary), 6 x H each, live in LDS at lane * 24 H; A_time of a
+// sample is formed where it is used (the same sums, in the same order, as the
+// reference's precomputed table).
+struct HbArgs {
+    const double* sol_r;
+    const double* sol_i;  // may be NULL
+    int64_t num_dofs;
+    int H;
+    const double* dmat_r;  // [Nt x H]
+    const double* dmat_i;  // may be NULL
+    int Nt;
+    const double* pmat_r;  // [H x Nt]
+    const double* pmat_i;  // may be NULL
+    const double* scalings;
+    int request_j;
+};
+
+__global__ void __launch_bounds__(kBlock)
+k_hbfem_coo(const double* __restrict__ P, int64_t nn, const double* __restrict__ T, int64_t ne,
+            const double* __restrict__ D, const double* __restrict__ S,
+            const double* __restrict__ Tags, HbArgs hb, QuadArg qa, MaterialView mat, int ntags,
+            double* __restrict__ Io, double* __restrict__ Jo, double* __restrict__ Vo,
+            double* __restrict__ Rrows, double* __restrict__ Rcols, double* __restrict__ RvR,
+            double* __restrict__ RvI, int* __restrict__ err) {
+#pragma clang fp contract(off)
+    extern __shared__ double hb_lds[];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int H = hb.H;
+    double* AR = hb_lds + (size_t)threadIdx.x * 24 * H;
+    double* AI = AR + 6 * H;
+    double* ReR = AR + 12 * H;
+    double* ReI = AR + 18 * H;
+    bool bad = false;
+    double dofs[6], s[6];
+    for (int k = 0; k < 6; ++k) {
+        dofs[k] = D[k + e * 6];
+        s[k] = S[k + e * 6];
+        const int gid = (int)dofs[k] - 1;
+        if (gid >= hb.num_dofs) bad = true;
+        for (int h = 0; h < H; ++h) {
+            const int64_t idx = (int64_t)gid + (int64_t)h * hb.num_dofs;
+            const bool ok = idx >= 0 && idx < hb.num_dofs * H;
+            AR[k + h * 6] = (ok ? hb.sol_r[idx] : 0.0) * s[k];
+            AI[k + h * 6] = hb.sol_i ? (ok ? hb.sol_i[idx] : 0.0) * s[k] : 0.0;
+            ReR[k + h * 6] = 0.0;
+            ReI[k + h * 6] = 0.0;
+        }
+    }
+    int tag = (int)Tags[e];
+    if (tag < 1 || tag > ntags) {  // UB in the reference; an index error here
+        bad = true;
+        tag = 1;
+    }
+    double p[12], J[9];
+    gather_nodes(P, nn, T, e, p, bad);
+    if (bad) atomicOr(err, 1);
+    const double detJ = jacobian_3d(p, J);
+    const double invDetJ = 1.0 / detJ;
+    const double absDetJ = fabs(detJ);
+    const bool imag_a = hb.sol_i && hb.dmat_i;
+    double Ke[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) Ke[i] = 0.0;
+    for (int q = 0; q < qa.nq; ++q) {
+        const double w = qa.qw[q] * absDetJ;
+        double C[18];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double cx = qa.ref[i + 0 + 18 * q], cy = qa.ref[i + 6 + 18 * q],
+                         cz = qa.ref[i + 12 + 18 * q];
+            C[i + 0] = (cx * J[0] + cy * J[1] + cz * J[2]) * invDetJ;
+            C[i + 6] = (cx * J[3] + cy * J[4] + cz * J[5]) * invDetJ;
+            C[i + 12] = (cx * J[6] + cy * J[7] + cz * J[8]) * invDetJ;
+        }
+        double sum_nu = 0.0;
+        for (int t = 0; t < hb.Nt; ++t) {
+            double Bx = 0.0, By = 0.0, Bz = 0.0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                double v = 0.0;
+                for (int h = 0; h < H; ++h) {
+                    double term = AR[i + h * 6] * hb.dmat_r[t + (int64_t)h * hb.Nt];
+                    if (imag_a) term -= AI[i + h * 6] * hb.dmat_i[t + (int64_t)h * hb.Nt];
+                    v += term;
+                }
+                Bx += C[i + 0] * v;
+                By += C[i + 6] * v;
+                Bz += C[i + 12] * v;
+            }
+            const double bsq = Bx * Bx + By * By + Bz * Bz;
+            double nu, dnu;
+            eval_nu(bsq, tag, mat, mat.brk, mat.coe, nu, dnu);
+            sum_nu += nu;
+            const double Hx = nu * Bx, Hy = nu * By, Hz = nu * Bz;
+            for (int h = 0; h < H; ++h) {
+                const double pr = hb.pmat_r[h + (int64_t)t * H];
+                const double pi = hb.pmat_i ? hb.pmat_i[h + (int64_t)t * H] : 0.0;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    const double dot = C[i + 0] * Hx + C[i + 6] * Hy + C[i + 12] * Hz;
+                    ReR[i + h * 6] += w * dot * pr;
+                    if (hb.pmat_i) ReI[i + h * 6] += w * dot * pi;
+                }
+            }
+        }
+        if (hb.request_j) {
+            const double nu_dc = sum_nu / (double)hb.Nt;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    const double dot =
+                        C[r + 0] * C[c + 0] + C[r + 6] * C[c + 6] + C[r + 12] * C[c + 12];
+                    Ke[r + c * 6] += w * nu_dc * dot;
+                }
+        }
+    }
+    for (int h = 0; h < H; ++h)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int64_t o = (int64_t)6 * H * e + 6 * h + k;
+            Rrows[o] = dofs[k];
+            Rcols[o] = (double)(h + 1);
+            RvR[o] = ReR[k + h * 6] * s[k];
+            RvI[o] = ReI[k + h * 6] * s[k];
+        }
+    if (hb.request_j) {
+        for (int h = 0; h < H; ++h) {
+            const double f = hb.scalings[h];
+            const double off = (double)((int64_t)h * hb.num_dofs);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    const int64_t o = (int64_t)36 * H * e + 36 * h + 6 * c + r;
+                    const double val = Ke[r + c * 6] * f;
+                    Io[o] = dofs[r] + off;
+                    Jo[o] = dofs[c] + off;
+                    Vo[o] = val * s[r] * s[c];
+                }
+        }
+    }
+}
+
 int check_quad(const mpk_quadrature* q) {
     MPK_REQUIRE(q != nullptr, MPK_ERR_ARGS, "quadrature table is NULL");
     MPK_REQUIRE(q->nq >= 1 && q->nq <= 8, MPK_ERR_ARGS, "nq=%d outside [1, 8]", q->nq);
@@ -829,6 +979,48 @@ int mpk_winding_coo(const double* P, int64_t nn, const double* T, int64_t ne,
         hipLaunchKernelGGL(k_winding_coo, dim3(grid_for(ne, kBlock)), dim3(kBlock), 0, st, P, nn,
                            T, ne, cell_dofs, signs, tags, (int)coil_id, dir_field, uniform_dir[0],
                            uniform_dir[1], uniform_dir[2], scale, qa, I_out, V_out, err);
+    });
+}
+
+int mpk_hbfem_coo(const double* P, int64_t nn, const double* T, int64_t ne,
+                  const double* cell_dofs, const double* signs, const double* tags,
+                  const double* sol_r, const double* sol_i, int64_t num_dofs, int32_t num_harm,
+                  const double* dmat_r, const double* dmat_i, int32_t num_time,
+                  const double* pmat_r, const double* pmat_i, const mpk_quadrature* quad,
+                  const mpk_material* mat, int request_j, const double* scalings,
+                  double* I_out, double* J_out, double* V_out, double* R_rows, double* R_cols,
+                  double* R_val_r, double* R_val_i, void* stream) {
+    clear_error();
+    if (int rc = check_quad(quad)) return rc;
+    MPK_REQUIRE(mat != nullptr, MPK_ERR_ARGS, "material table is NULL");
+    MPK_REQUIRE(ne >= 0 && nn >= 0 && num_dofs >= 0, MPK_ERR_ARGS, "negative size");
+    MPK_REQUIRE(num_harm >= 1 && num_harm <= kHbMaxHarm, MPK_ERR_LIMIT,
+                "%d harmonics outside [1, %d]", num_harm, kHbMaxHarm);
+    MPK_REQUIRE(num_time >= 1, MPK_ERR_ARGS, "no time samples");
+    if (ne == 0) return MPK_OK;
+    MPK_REQUIRE(P && T && cell_dofs && signs && tags && sol_r && dmat_r && pmat_r && R_rows &&
+                    R_cols && R_val_r && R_val_i,
+                MPK_ERR_ARGS, "NULL array argument");
+    MPK_REQUIRE(!request_j || (I_out && J_out && V_out && scalings), MPK_ERR_ARGS,
+                "NULL Jacobian output or scalings");
+    MPK_REQUIRE(nn >= 1, MPK_ERR_INDEX, "elements given but no nodes");
+    MPK_REQUIRE(mat->num_tags >= 1 && mat->linear_nu && mat->is_nonlinear && mat->max_bsq &&
+                    mat->spline_start && mat->spline_count && mat->coef_start,
+                MPK_ERR_ARGS, "material table without tags");
+    MaterialView mv{mat->linear_nu, mat->is_nonlinear, mat->max_bsq,   mat->breaks,
+                    mat->coefs,     mat->spline_start, mat->spline_count, mat->coef_start};
+    HbArgs hb{sol_r, sol_i, num_dofs, num_harm, dmat_r, dmat_i, num_time, pmat_r, pmat_i,
+              scalings, request_j ? 1 : 0};
+    const QuadArg qa = make_quad_arg(quad, true);
+    // lanes per block so that 24 H doubles per lane fit in 64 KB of LDS
+    int lanes = kBlock;
+    while (lanes > 1 && (size_t)lanes * 24 * num_harm * sizeof(double) > 65536) lanes /= 2;
+    const size_t lds = (size_t)lanes * 24 * num_harm * sizeof(double);
+    hipStream_t st = as_stream(stream);
+    return run_checked(st, [&](int* err) {
+        hipLaunchKernelGGL(k_hbfem_coo, dim3(grid_for(ne, lanes)), dim3(lanes), lds, st, P, nn, T,
+                           ne, cell_dofs, signs, tags, hb, qa, mv, (int)mat->num_tags, I_out,
+                           J_out, V_out, R_rows, R_cols, R_val_r, R_val_i, err);
     });
 }
 

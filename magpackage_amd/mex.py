@@ -19,6 +19,12 @@ identifier as the MEX files that ``Assembler.m`` calls:
 - ``assemble_winding_kernel_mex(P, T, Dofs, Signs, RegionTags, CoilID,
   DirField, UniformDir, Scale, Qw, RefVal)``
   src/mex/assemble_winding_kernel_mex.cpp:13-126 (11 inputs, DirField may be [])
+- ``assemble_hbfem_kernel_mex(P, T, Dofs, Signs, Tags, SolH_Real, SolH_Imag,
+  Dmat_Real, Dmat_Imag, Pmat_Real, Pmat_Imag, Qw, CurlRef, linNu, isNon,
+  maxBSq, breaks, coefs, starts, counts, RequestJacobian, Scalings[,
+  coeStarts])`` src/mex/assemble_hbfem_kernel_mex.cpp:25-260 (22 inputs; the
+  MEX reads an uninitialised coefficient-offset table, here it is the optional
+  23rd input or derived as packMaterialData builds it)
 - ``sparse_gpu(I, J, V, m, n)``: ``valid = I>0 & J>0; sparse(...)``
   (src/Assembler.m:37-38) on the GPU.
 
@@ -214,6 +220,52 @@ def assemble_winding_kernel_mex(*args):
         float(np.asarray(CoilID).ravel()[0]), L.ptr(dirf), uni[:3].ctypes.data_as(C.c_void_p),
         float(np.asarray(Scale).ravel()[0]), C.byref(q), L.ptr(Io), L.ptr(Vo), L.stream_ptr()))
     return tuple(t[: 6 * ne].cpu().numpy() for t in (Io, Vo))
+
+
+def assemble_hbfem_kernel_mex(*args):
+    """[I, J, V, R_rows, R_cols, R_val_Real, R_val_Imag] = assemble_hbfem_kernel_mex(...)
+    (src/mex/assemble_hbfem_kernel_mex.cpp:25-260).  Imaginary parts may be []."""
+    if len(args) < 22:
+        raise L.MpkError(L.MPK_ERR_ARGS, "Need 22 inputs (Split Complex).")
+    (P, T, Dofs, Signs, Tags, SolR, SolI, DmR, DmI, PmR, PmI, Qw, CurlRef, linNu, isNon,
+     maxB, breaks, coefs, starts, counts, reqJ, Scalings) = args[:22]
+    ne = _ncols(T)
+    nn = _colmajor(P).size // 3
+    sol = np.asarray(SolR, dtype=np.float64)
+    if sol.ndim == 1:
+        sol = sol[:, None]
+    nd, H = sol.shape
+    nt = _colmajor(DmR).size // H  # Dmat_Real is [Nt x H]
+    cnt = _colmajor(counts)
+    if len(args) >= 23:
+        coe = _colmajor(args[22])
+    else:
+        # packMaterialData (src/Assembler.m:270-304): coefficient blocks of the
+        # nonlinear tags in ascending tag order, 4 per spline piece
+        coe = np.concatenate([[0.0], np.cumsum(4.0 * cnt)[:-1]]) if cnt.size else cnt
+    q = _quad(Qw, CurlRef, True)
+    d = [_dev(_colmajor(x)) for x in (P, T, Dofs, Signs, Tags)]
+    opt = lambda x: _dev(_colmajor(x)) if np.size(x) else None
+    sr, si = _dev(_colmajor(sol)), opt(SolI)
+    dr, di, pr, pi_ = opt(DmR), opt(DmI), opt(PmR), opt(PmI)
+    mats = [_dev(_colmajor(x)) for x in (linNu, isNon, maxB, breaks, coefs, starts, cnt, coe)]
+    mat = L.Material(linear_nu=L.ptr(mats[0]), is_nonlinear=L.ptr(mats[1]), max_bsq=L.ptr(mats[2]),
+                     breaks=L.ptr(mats[3]), coefs=L.ptr(mats[4]), spline_start=L.ptr(mats[5]),
+                     spline_count=L.ptr(mats[6]), coef_start=L.ptr(mats[7]),
+                     num_tags=int(_colmajor(linNu).size), num_breaks=int(_colmajor(breaks).size),
+                     num_coefs=int(_colmajor(coefs).size))
+    request_j = bool(np.asarray(reqJ).ravel()[0])
+    sc = _dev(_colmajor(Scalings))
+    nk = 36 * H * ne if request_j else 0
+    Io, Jo, Vo = _out(nk), _out(nk), _out(nk)
+    R = [_out(6 * H * ne) for _ in range(4)]
+    L.check(L.lib.mpk_hbfem_coo(
+        L.ptr(d[0]), nn, L.ptr(d[1]), ne, L.ptr(d[2]), L.ptr(d[3]), L.ptr(d[4]), L.ptr(sr),
+        L.ptr(si), nd, H, L.ptr(dr), L.ptr(di), nt, L.ptr(pr), L.ptr(pi_), C.byref(q),
+        C.byref(mat), int(request_j), L.ptr(sc), L.ptr(Io), L.ptr(Jo), L.ptr(Vo), L.ptr(R[0]),
+        L.ptr(R[1]), L.ptr(R[2]), L.ptr(R[3]), L.stream_ptr()))
+    return (Io[:nk].cpu().numpy(), Jo[:nk].cpu().numpy(), Vo[:nk].cpu().numpy(),
+            *(t[: 6 * H * ne].cpu().numpy() for t in R))
 
 
 def sparse_gpu(I, J, V, m, n, drop_zeros: bool = True):

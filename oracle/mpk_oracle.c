@@ -25,6 +25,9 @@
  *   orc_scalar_laplacian_kernel  src/mex/assemble_scalar_laplacian_kernel_mex.cpp:13-105
  *   orc_coupling_kernel  src/mex/assemble_coupling_kernel_mex.cpp:13-131
  *   orc_winding_kernel   src/mex/assemble_winding_kernel_mex.cpp:13-126
+ *   orc_hbfem_kernel     src/mex/assemble_hbfem_kernel_mex.cpp:25-260 (the spline
+ *                        coefficient offsets passed explicitly: the MEX leaves
+ *                        its coeStart uninitialised)
  *   orc_element_b        src/post/PostProcessor.m:64-126 (computeElementB)
  *   orc_sparse           src/Assembler.m:37-38  (valid = I>0 & J>0; MATLAB sparse():
  *                        duplicates summed, CSC with sorted rows, zeros dropped;
@@ -735,5 +738,139 @@ void orc_winding_kernel(const double* P, const double* T, const double* Dofs,
             I[6 * e + i] = (double)(long)Dofs[i + 6 * e];
             V[6 * e + i] = F[i] * Signs[i + 6 * e];
         }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* HBFEM harmonic-balance residual and block Jacobian                  */
+/* ------------------------------------------------------------------ */
+/*
+ * SolR/SolI [numDofs x H] (SolI may be NULL), DmatR/DmatI [Nt x H] (DmatI may
+ * be NULL), PmatR/PmatI [H x Nt] (PmatI may be NULL), column-major.  Per element:
+ * A_loc = Sol(dof, :) s (a DOF of 0 reads Sol at index h*numDofs - 1 for
+ * h >= 1, as the MEX does, and 0 for h = 0, where the MEX reads before the
+ * array); A_time = Re(A_loc Dmat^T) summed over h in order; per quadrature
+ * point and time sample B = C A_time, nu(|B|^2), H = nu B,
+ * Re(:, h) += w (C . H) Pmat(h, t); Ke += w mean_t(nu) C C^T.  Outputs
+ * R rows/cols/real/imag at 6 H e + 6 h + k and, with requestJ, 36 H e triplets
+ * (dof + h numDofs, block Ke Scalings[h] s_r s_c) at 36 H e + 36 h + 6 c + r.
+ */
+void orc_hbfem_kernel(const double* P, const double* T, const double* Dofs,
+                      const double* Signs, const double* Tags, const double* SolR,
+                      const double* SolI, int64_t numDofs, int64_t H, const double* DmatR,
+                      const double* DmatI, int64_t Nt, const double* PmatR,
+                      const double* PmatI, const double* Qw, int64_t nq, const double* RefCurl,
+                      const double* lin, const double* isNon, const double* maxB,
+                      const double* brk, const double* coe, const double* start,
+                      const double* cnt, const double* coeStart, int requestJ,
+                      const double* Scalings, int64_t ne, double* I, double* Jo, double* V,
+                      double* Rrows, double* Rcols, double* RvalR, double* RvalI) {
+    orc_material mat = {lin, isNon, maxB, brk, coe, start, cnt, coeStart};
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < ne; e++) {
+        double* AR = (double*)malloc(sizeof(double) * 6 * H);
+        double* AI = (double*)malloc(sizeof(double) * 6 * H);
+        double* At = (double*)malloc(sizeof(double) * 6 * Nt);
+        double* ReR = (double*)calloc((size_t)(6 * H), sizeof(double));
+        double* ReI = (double*)calloc((size_t)(6 * H), sizeof(double));
+        double dofs[6], s[6], Ke[36];
+        memset(Ke, 0, sizeof(Ke));
+        for (int k = 0; k < 6; k++) {
+            dofs[k] = Dofs[k + e * 6];
+            s[k] = Signs[k + e * 6];
+            int gid = (int)dofs[k] - 1;
+            for (int64_t h = 0; h < H; h++) {
+                int64_t idx = gid + h * numDofs;
+                double vr = idx >= 0 ? SolR[idx] : 0.0;
+                AR[k + h * 6] = vr * s[k];
+                if (SolI) {
+                    double vi = idx >= 0 ? SolI[idx] : 0.0;
+                    AI[k + h * 6] = vi * s[k];
+                } else {
+                    AI[k + h * 6] = 0.0;
+                }
+            }
+        }
+        for (int64_t t = 0; t < Nt; t++)
+            for (int k = 0; k < 6; k++) {
+                double sum = 0;
+                for (int64_t h = 0; h < H; h++) {
+                    double term = AR[k + h * 6] * DmatR[t + h * Nt];
+                    if (SolI && DmatI) term -= AI[k + h * 6] * DmatI[t + h * Nt];
+                    sum += term;
+                }
+                At[k + t * 6] = sum;
+            }
+        int tag = (int)Tags[e];
+        double p[12], J[9];
+        orc_gather(P, T, e, p);
+        double detJ = orc_jacobian_3d(p, J);
+        double invDetJ = 1.0 / detJ;
+        double absDetJ = fabs(detJ);
+        for (int64_t q = 0; q < nq; q++) {
+            double w = Qw[q] * absDetJ;
+            double Cp[18];
+            for (int i = 0; i < 6; i++) {
+                double cx = RefCurl[i + 0 + q * 18], cy = RefCurl[i + 6 + q * 18],
+                       cz = RefCurl[i + 12 + q * 18];
+                Cp[i + 0] = (cx * J[0] + cy * J[1] + cz * J[2]) * invDetJ;
+                Cp[i + 6] = (cx * J[3] + cy * J[4] + cz * J[5]) * invDetJ;
+                Cp[i + 12] = (cx * J[6] + cy * J[7] + cz * J[8]) * invDetJ;
+            }
+            double sum_nu = 0;
+            for (int64_t t = 0; t < Nt; t++) {
+                double Bx = 0, By = 0, Bz = 0;
+                for (int i = 0; i < 6; i++) {
+                    double v = At[i + t * 6];
+                    Bx += Cp[i + 0] * v;
+                    By += Cp[i + 6] * v;
+                    Bz += Cp[i + 12] * v;
+                }
+                double bsq = Bx * Bx + By * By + Bz * Bz;
+                double nu, dnu;
+                orc_eval_nu(bsq, tag, &mat, &nu, &dnu);
+                sum_nu += nu;
+                double Hx = nu * Bx, Hy = nu * By, Hz = nu * Bz;
+                for (int64_t h = 0; h < H; h++) {
+                    double pr = PmatR[h + t * H];
+                    for (int i = 0; i < 6; i++) {
+                        double dot = Cp[i + 0] * Hx + Cp[i + 6] * Hy + Cp[i + 12] * Hz;
+                        ReR[i + h * 6] += w * dot * pr;
+                        if (PmatI) ReI[i + h * 6] += w * dot * PmatI[h + t * H];
+                    }
+                }
+            }
+            if (requestJ) {
+                double nu_dc = sum_nu / (double)Nt;
+                for (int c = 0; c < 6; c++)
+                    for (int r = 0; r < 6; r++) {
+                        double dot = Cp[r + 0] * Cp[c + 0] + Cp[r + 6] * Cp[c + 6] +
+                                     Cp[r + 12] * Cp[c + 12];
+                        Ke[r + c * 6] += w * nu_dc * dot;
+                    }
+            }
+        }
+        for (int64_t h = 0; h < H; h++)
+            for (int k = 0; k < 6; k++) {
+                long o = 6 * H * e + 6 * h + k;
+                Rrows[o] = dofs[k];
+                Rcols[o] = (double)(h + 1);
+                RvalR[o] = ReR[k + h * 6] * s[k];
+                RvalI[o] = ReI[k + h * 6] * s[k];
+            }
+        if (requestJ) {
+            for (int64_t h = 0; h < H; h++) {
+                double f = Scalings[h];
+                for (int c = 0; c < 6; c++)
+                    for (int r = 0; r < 6; r++) {
+                        long o = 36 * H * e + 36 * h + 6 * c + r;
+                        double val = Ke[r + c * 6] * f;
+                        I[o] = dofs[r] + (double)(h * numDofs);
+                        Jo[o] = dofs[c] + (double)(h * numDofs);
+                        V[o] = val * s[r] * s[c];
+                    }
+            }
+        }
+        free(AR); free(AI); free(At); free(ReR); free(ReI);
     }
 }

@@ -175,6 +175,33 @@ def winding(P, T, Dofs, Signs, Tags, CoilID, DirField, UniDir, Scale, Qw, RefVal
     return I, V
 
 
+def hbfem(P, T, Dofs, Signs, Tags, SolR, SolI, DmatR, DmatI, PmatR, PmatI, Qw, CurlRef, packed,
+          requestJ, Scalings):
+    """assemble_hbfem_kernel_mex restated: Sol [numDofs x H], Dmat [Nt x H],
+    Pmat [H x Nt] (real and imaginary parts; an imaginary part may be None).
+    Returns (I, J, V, R_rows, R_cols, R_val_real, R_val_imag)."""
+    SolR = np.asarray(SolR, dtype=np.float64)
+    nd, H = SolR.shape
+    Nt = np.asarray(DmatR).shape[0]
+    f = lambda a: _d(np.asarray(a, dtype=np.float64).ravel(order="F")) if a is not None else (None, None)
+    Pp = _d(P); Tp = _d(T); Dp = _d(Dofs); Sp = _d(Signs); Gp = _d(Tags)
+    sr, si, dr, di, pr, pi_ = f(SolR), f(SolI), f(DmatR), f(DmatI), f(PmatR), f(PmatI)
+    Qp = _d(Qw); Cp = _d(CurlRef); Scp = _d(Scalings)
+    arrs = [_d(packed[k]) for k in ("linNu", "isNon", "maxB", "breaks", "coefs", "starts",
+                                    "counts", "coe_starts")]
+    ne = Tp[0].size // 4
+    nk = 36 * H * ne if requestJ else 0
+    I = np.zeros(max(nk, 1)); J = np.zeros(max(nk, 1)); V = np.zeros(max(nk, 1))
+    Rr, Rc, Rvr, Rvi = (np.zeros(6 * H * ne) for _ in range(4))
+    _lib.orc_hbfem_kernel(Pp[1], Tp[1], Dp[1], Sp[1], Gp[1], sr[1], si[1], _i64(nd), _i64(H),
+                          dr[1], di[1], _i64(Nt), pr[1], pi_[1], Qp[1], _i64(Qp[0].size), Cp[1],
+                          *[a[1] for a in arrs], C.c_int(1 if requestJ else 0), Scp[1], _i64(ne),
+                          I.ctypes.data_as(_dp), J.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
+                          Rr.ctypes.data_as(_dp), Rc.ctypes.data_as(_dp),
+                          Rvr.ctypes.data_as(_dp), Rvi.ctypes.data_as(_dp))
+    return I[:nk], J[:nk], V[:nk], Rr, Rc, Rvr, Rvi
+
+
 def element_b(P, T, Dofs, Signs, sol, CurlRef):
     """PostProcessor.computeElementB for one solution column: B (Ne, 3)."""
     Pp = _d(P); Tp = _d(T); Dp = _d(Dofs); Sp = _d(Signs); sp = _d(sol); Cp = _d(CurlRef)

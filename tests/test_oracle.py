@@ -478,3 +478,37 @@ def test_winding_closed_form_random_tets():
             assert np.array_equal(I[blk], D[e])
             worst = max(worst, rel_err(V[blk], ref))
         assert worst < 1e-12
+
+
+def test_hbfem_linear_closed_form():
+    """For a linear material the harmonic-balance residual of harmonic h is
+    K(nu) A_h (the DFT of the periodic field at the samples recovers each
+    harmonic, Nt > 2 max h), and the Jacobian blocks are K(nu) scaled by 1
+    (DC) or 2 (AC)."""
+    from magpackage_amd.hbfem import AFT
+    mesh = kuhn_box(3, 3, 2, seed=41)
+    mi = O.mesh_inputs(mesh)
+    n = mi["ndof"]
+    aft = AFT([0, 1, 3], 16)
+    H = aft.NumHarmonics
+    rng = np.random.default_rng(42)
+    A = rng.normal(size=(n, H)) + 1j * rng.normal(size=(n, H))
+    A[:, 0] = A[:, 0].real  # the DC coefficient is real
+    lib = {1: create_linear(1.0), 2: create_linear(250.0)}
+    tags = np.where(rng.uniform(size=mesh.num_elems) < 0.5, 1.0, 2.0)
+    packed = pack_material_data(lib)
+    qw, curl, _ = stiffness_tables()
+    sc = aft.jacobian_scalings()
+    I, J, V, Rr, Rc, Rvr, Rvi = O.hbfem(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, A.real,
+                                        A.imag, aft.D_matrix.real, aft.D_matrix.imag,
+                                        aft.P_matrix.real, aft.P_matrix.imag, qw, curl, packed,
+                                        True, sc)
+    nu = np.where(tags == 2.0, packed["linNu"][1], packed["linNu"][0])
+    K = O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl), n, n)
+    R = np.zeros((n, H), dtype=complex)
+    np.add.at(R, (Rr.astype(int) - 1, Rc.astype(int) - 1), Rvr + 1j * Rvi)
+    ref = K @ A
+    assert np.abs(R - ref).max() / np.abs(ref).max() < 1e-12
+    Jm = sp.csc_matrix((V, (I.astype(int) - 1, J.astype(int) - 1)), shape=(n * H, n * H))
+    Jref = sp.block_diag([K * f for f in sc]).tocsc()
+    assert abs(Jm - Jref).max() / abs(Jref).max() < 1e-12
