@@ -21,14 +21,14 @@ def mass_tables(order=2):
     return q_w, matlab_flat(val), q_pts
 
 
-def config1_mesh(n=20, seed=1):
-    """Config-1 style mesh: unit Kuhn box, iron core [0.3,0.7]^3 (tag 2) in air (tag 1)."""
+def core_box_mesh(n=20, seed=1):
+    """Unit Kuhn box of n^3 cubes, iron core [0.3,0.7]^3 (tag 2) in air (tag 1)."""
     m = kuhn_box(n, n, n, seed=seed)
     tag_box_region(m, (0.3, 0.3, 0.3), (0.7, 0.7, 0.7), 2)
     return m
 
 
-def config1_nu(mesh):
+def core_box_nu(mesh):
     nu = np.where(mesh.RegionTags == 2, 1.0 / (1000.0 * MU0), 1.0 / MU0)
     return nu.astype(np.float64)
 

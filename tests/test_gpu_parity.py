@@ -12,7 +12,7 @@ import pytest
 from oracle import oracle as O
 from magpackage_amd.material import create_linear, marrocco, pack_material_data, MU0
 from magpackage_amd.meshgen import kuhn_box, kuhn_counts
-from tests.helpers import (config1_mesh, config1_nu, csc_equal_bits, csc_from_plan, element_B,
+from tests.helpers import (core_box_mesh, core_box_nu, csc_equal_bits, csc_from_plan, element_B,
                            gradient_matrix, mass_tables, random_tets, rel_err, stiffness_tables)
 
 pytestmark = pytest.mark.gpu
@@ -37,7 +37,7 @@ def _setup(mesh, gpu, active_tags=None):
 
 # ------------------------------------------------------------ topology -----
 def test_generate_edges_and_dofs_bit_exact(gpu):
-    mesh = config1_mesh(12, seed=2)
+    mesh = core_box_mesh(12, seed=2)
     M, dh, space, asm = _setup(mesh, gpu)
     edges, t2e, sign = O.generate_edges(mesh.T)
     assert np.array_equal(M.Edges.cpu().numpy(), edges)
@@ -49,7 +49,7 @@ def test_generate_edges_and_dofs_bit_exact(gpu):
 
 
 def test_distribute_dofs_restricted_bit_exact(gpu):
-    mesh = config1_mesh(8, seed=3)
+    mesh = core_box_mesh(8, seed=3)
     M, dh, space, asm = _setup(mesh, gpu, active_tags=[2])
     edges, t2e, _ = O.generate_edges(mesh.T)
     emap, cell, nd = O.distribute_dofs(t2e, edges.shape[0], mesh.RegionTags == 2)
@@ -76,13 +76,13 @@ def test_coo_kernels_bit_exact_kat(gpu):
 @pytest.mark.parametrize("ne_cubes", [(1, 1, 1), (7, 5, 3), (20, 20, 20)])
 def test_coo_kernels_bit_exact_mesh(gpu, ne_cubes):
     from magpackage_amd import mex
-    mesh = config1_mesh(1) if ne_cubes == (1, 1, 1) else kuhn_box(*ne_cubes, seed=4)
+    mesh = core_box_mesh(1) if ne_cubes == (1, 1, 1) else kuhn_box(*ne_cubes, seed=4)
     if ne_cubes == (20, 20, 20):
-        mesh = config1_mesh(20, seed=1)
+        mesh = core_box_mesh(20, seed=1)
     mi = O.mesh_inputs(mesh)
     rng = np.random.default_rng(5)
     ne = mesh.num_elems
-    nu = config1_nu(mesh) if ne_cubes == (20, 20, 20) else rng.uniform(1, 2, ne)
+    nu = core_box_nu(mesh) if ne_cubes == (20, 20, 20) else rng.uniform(1, 2, ne)
     qw, curl, _ = stiffness_tables()
     args = (mi["P"].T, mi["T"].T, mi["Dofs"].T, mi["Signs"].T)
     I, J, V = mex.assemble_curl_curl_kernel_mex(*args, nu, qw, curl)
@@ -127,7 +127,7 @@ def test_nodal_and_winding_coo_bit_exact(gpu):
     the oracle, bit for bit (a zero-sigma region, inactive node DOFs, both
     DirField forms, elements outside the coil)."""
     from magpackage_amd import mex
-    mesh = config1_mesh(6, seed=9)
+    mesh = core_box_mesh(6, seed=9)
     mi = O.mesh_inputs(mesh)
     ne = mesh.num_elems
     rng = np.random.default_rng(40)
@@ -177,7 +177,7 @@ def test_hbfem_coo_bit_exact(gpu, request_j):
     coefficient offsets derived as packMaterialData builds them."""
     from magpackage_amd import mex
     from magpackage_amd.hbfem import AFT
-    mesh = config1_mesh(6, seed=12)
+    mesh = core_box_mesh(6, seed=12)
     mi = O.mesh_inputs(mesh)
     n = mi["ndof"]
     aft = AFT([0, 1, 3, 5], 24)
@@ -208,7 +208,7 @@ def test_assemble_hbfem_linear_and_nonlinear(gpu):
     oracle's triplets summed per (DOF, harmonic)."""
     import scipy.sparse as sps
     from magpackage_amd.hbfem import AFT
-    mesh = config1_mesh(6, seed=13)
+    mesh = core_box_mesh(6, seed=13)
     M_, dh, space, asm = _setup(mesh, gpu)
     mi = O.mesh_inputs(mesh)
     n = mi["ndof"]
@@ -274,7 +274,7 @@ def test_sparse_coo_to_csc_bit_exact(gpu):
 # ------------------------------------------------------------ CSC engine ---
 @pytest.fixture(scope="module")
 def cfg1(gpu):
-    mesh = config1_mesh(20, seed=1)
+    mesh = core_box_mesh(20, seed=1)
     M, dh, space, asm = _setup(mesh, gpu)
     mi = O.mesh_inputs(mesh)
     return mesh, M, dh, space, asm, mi
@@ -310,7 +310,7 @@ def test_plan_pattern_matches_sparse(cfg1):
 def test_csc_stiffness(cfg1, prec, driver):
     _one_exact_driver(prec, driver)
     mesh, M, dh, space, asm, mi = cfg1
-    nu = config1_nu(mesh)
+    nu = core_box_nu(mesh)
     qw, curl, _ = stiffness_tables()
     ref = O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl),
                          mi["ndof"], mi["ndof"])
@@ -394,7 +394,7 @@ def test_csc_preloaded_coordinates(cfg1, prec, driver):
     from magpackage_amd import _lib as L
     from magpackage_amd.assembler import Plan
     mesh, M, dh, space, asm, mi = cfg1
-    nu = torch.from_numpy(config1_nu(mesh)).cuda()
+    nu = torch.from_numpy(core_box_nu(mesh)).cuda()
     q = asm.stiffness_quadrature()
     pr = L.PRECISIONS[prec]
     fresh = Plan(M, dh.DofMaps[space.toString()], dh.NumGlobalDofs)
@@ -415,7 +415,7 @@ def test_csc_preloaded_coordinates(cfg1, prec, driver):
 
 
 def test_mass_weighted_empty_when_sigma_zero(cfg1):
-    """Config 1 has sigma = 0 everywhere: sparse() of the weighted mass is empty."""
+    """With sigma = 0 everywhere, sparse() of the weighted mass is empty."""
     mesh, M, dh, space, asm, mi = cfg1
     Mg = asm.assembleMassWeighted(space, np.zeros(mesh.num_elems), precision="fp64_exact")
     assert Mg.nnz == 0
@@ -425,7 +425,7 @@ def test_mass_weighted_empty_when_sigma_zero(cfg1):
 def test_csc_frequency_operator(cfg1, prec, driver):
     _one_exact_driver(prec, driver)
     mesh, M, dh, space, asm, mi = cfg1
-    nu = config1_nu(mesh)
+    nu = core_box_nu(mesh)
     sig = np.where(mesh.RegionTags == 2, 3.5e7, 0.0)
     omega = 2 * np.pi * 50.0
     qw, curl, _ = stiffness_tables()
@@ -573,7 +573,7 @@ def test_newton_set_tile_classes(gpu, case, overlap, monkeypatch):
         mesh = kuhn_box(2, 2, 2, seed=7)
         tag_box_region(mesh, (0.0, 0.0, 0.0), (0.5, 1.0, 1.0), 2)
     else:
-        mesh = config1_mesh(10, seed=5)
+        mesh = core_box_mesh(10, seed=5)
     M_, dh, space, asm = _setup(mesh, gpu)
     mi = O.mesh_inputs(mesh)
     n = mi["ndof"]

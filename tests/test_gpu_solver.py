@@ -7,7 +7,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from oracle import oracle as O
-from tests.helpers import config1_mesh, config1_nu
+from tests.helpers import core_box_mesh, core_box_nu
 
 pytestmark = pytest.mark.gpu
 
@@ -16,13 +16,13 @@ pytestmark = pytest.mark.gpu
 def system(gpu):
     import torch
     from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
-    km = config1_mesh(8, seed=30)
+    km = core_box_mesh(8, seed=30)
     mesh = Mesh.from_kuhn(km)
     dh = DofHandler(mesh)
     space = FunctionSpace("Nedelec", 1)
     dh.distributeDofs(space)
     asm = Assembler(mesh, dh)
-    K = asm.assembleStiffness(space, config1_nu(km), drop_zeros=False)
+    K = asm.assembleStiffness(space, core_box_nu(km), drop_zeros=False)
     M = asm.assembleMass(space, drop_zeros=False)
     edges = mesh.Edges.cpu().numpy()
     fixed = O.boundary_edges(km.T, edges)

@@ -16,7 +16,7 @@ from magpackage_amd.fem_tables import EDGE_NODES, barycentric_moments
 from magpackage_amd.material import (MU0, create_linear, create_nonlinear, marrocco,
                                      pack_material_data)
 from magpackage_amd.meshgen import kuhn_box, kuhn_counts
-from tests.helpers import (config1_mesh, config1_nu, gradient_matrix, grads, mass_tables,
+from tests.helpers import (core_box_mesh, core_box_nu, gradient_matrix, grads, mass_tables,
                            random_tets, rel_err, stiffness_tables, whitney_curls)
 
 REF_CURLS = np.array([[0, -2, 2], [2, 0, -2], [-2, 2, 0], [0, 0, 2], [0, -2, 0], [2, 0, 0]],
@@ -130,9 +130,9 @@ def test_mass_sigma_rules():
 # --------------------------------------------------- global invariants -----
 @pytest.fixture(scope="module")
 def small_system():
-    m = config1_mesh(6, seed=11)
+    m = core_box_mesh(6, seed=11)
     mi = O.mesh_inputs(m)
-    nu = config1_nu(m)
+    nu = core_box_nu(m)
     qw, curl, _ = stiffness_tables()
     I, J, V = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl)
     n = mi["ndof"]
@@ -165,7 +165,7 @@ def test_stiffness_invariants(small_system):
 
 
 def test_mass_spd():
-    m = config1_mesh(5, seed=13)
+    m = core_box_mesh(5, seed=13)
     mi = O.mesh_inputs(m)
     qw, val, _ = mass_tables(2)
     I, J, V = O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], None, qw, val)
@@ -176,14 +176,14 @@ def test_mass_spd():
 
 
 # ---------------------------------------------------------- Jacobian --------
-def _mat_lib_config1():
+def _mat_lib_core_box():
     return {1: create_linear(1.0), 2: create_linear(1000.0)}
 
 
 def test_jacobian_linear_equals_stiffness(small_system):
     """Linear materials: J == K bitwise and R == K a (MexMaterialUtils.hpp:30-34)."""
     m, mi, nu, K = small_system
-    packed = pack_material_data(_mat_lib_config1())
+    packed = pack_material_data(_mat_lib_core_box())
     qw, curl, _ = stiffness_tables()
     n = mi["ndof"]
     a = np.random.default_rng(14).normal(size=n)
@@ -204,7 +204,7 @@ def test_jacobian_linear_equals_stiffness(small_system):
 
 def test_jacobian_is_residual_derivative():
     """Nonlinear Marrocco iron: J = dR/da by central differences of the residual mode."""
-    m = config1_mesh(4, seed=15)
+    m = core_box_mesh(4, seed=15)
     mi = O.mesh_inputs(m)
     lib = {1: create_linear(1.0), 2: marrocco()}
     packed = pack_material_data(lib)
@@ -333,7 +333,7 @@ def test_boundary_edges_brute_force():
 
 
 def test_distribute_dofs_restricted():
-    mesh = config1_mesh(4, seed=21)
+    mesh = core_box_mesh(4, seed=21)
     edges, t2e, _ = O.generate_edges(mesh.T)
     act = mesh.RegionTags == 2
     emap, cell, nd = O.distribute_dofs(t2e, edges.shape[0], act, offset=7)
