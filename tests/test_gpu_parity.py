@@ -273,7 +273,7 @@ def test_sparse_coo_to_csc_bit_exact(gpu):
 
 # ------------------------------------------------------------ CSC engine ---
 @pytest.fixture(scope="module")
-def cfg1(gpu):
+def core20(gpu):
     mesh = core_box_mesh(20, seed=1)
     M, dh, space, asm = _setup(mesh, gpu)
     mi = O.mesh_inputs(mesh)
@@ -292,8 +292,8 @@ def _one_exact_driver(prec, driver):
         pytest.skip("the exact mode has a single (element-order) driver")
 
 
-def test_plan_pattern_matches_sparse(cfg1):
-    mesh, M, dh, space, asm, mi = cfg1
+def test_plan_pattern_matches_sparse(core20):
+    mesh, M, dh, space, asm, mi = core20
     plan = asm.plan(space)
     qw, curl, _ = stiffness_tables()
     I, J, V = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], np.ones(mesh.num_elems),
@@ -307,9 +307,9 @@ def test_plan_pattern_matches_sparse(cfg1):
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
-def test_csc_stiffness(cfg1, prec, driver):
+def test_csc_stiffness(core20, prec, driver):
     _one_exact_driver(prec, driver)
-    mesh, M, dh, space, asm, mi = cfg1
+    mesh, M, dh, space, asm, mi = core20
     nu = core_box_nu(mesh)
     qw, curl, _ = stiffness_tables()
     ref = O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl),
@@ -325,9 +325,9 @@ def test_csc_stiffness(cfg1, prec, driver):
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
-def test_csc_mass(cfg1, prec, driver):
+def test_csc_mass(core20, prec, driver):
     _one_exact_driver(prec, driver)
-    mesh, M, dh, space, asm, mi = cfg1
+    mesh, M, dh, space, asm, mi = core20
     qm, val, _ = mass_tables(2)
     rng = np.random.default_rng(7)
     sig = np.where(mesh.RegionTags == 2, 3.5e7, 0.0)
@@ -344,11 +344,11 @@ def test_csc_mass(cfg1, prec, driver):
             assert abs(Mg - ref).max() / abs(ref).max() < TOL[prec]
 
 
-def test_source_vector_bit_exact(cfg1):
+def test_source_vector_bit_exact(core20):
     """assemble_source_kernel_mex (COO) and assembleSource (per-DOF sums in
     triplet order) against the oracle + sparse(I, 1, V, N, 1), bitwise."""
     from magpackage_amd.mex import assemble_source_kernel_mex
-    mesh, M, dh, space, asm, mi = cfg1
+    mesh, M, dh, space, asm, mi = core20
     qm, val, _ = mass_tables(2)
     rng = np.random.default_rng(31)
     Js = rng.normal(size=(mesh.num_elems, 3)) * 3e6
@@ -375,8 +375,8 @@ def test_source_vector_bit_exact(cfg1):
     assert np.array_equal(F2.view(np.uint64), F2ref.view(np.uint64))
 
 
-def test_element_b_bit_exact(cfg1):
-    mesh, M, dh, space, asm, mi = cfg1
+def test_element_b_bit_exact(core20):
+    mesh, M, dh, space, asm, mi = core20
     _, curl, _ = stiffness_tables()
     a = np.random.default_rng(32).normal(size=mi["ndof"])
     Bref = O.element_b(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a, curl)
@@ -385,7 +385,7 @@ def test_element_b_bit_exact(cfg1):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
-def test_csc_preloaded_coordinates(cfg1, prec, driver):
+def test_csc_preloaded_coordinates(core20, prec, driver):
     """P == NULL reads the coordinates loaded by mpk_plan_set_coords: values
     are bit-identical to the call given P; FP64_EXACT and a plan that never
     saw coordinates reject P == NULL."""
@@ -393,7 +393,7 @@ def test_csc_preloaded_coordinates(cfg1, prec, driver):
     import torch
     from magpackage_amd import _lib as L
     from magpackage_amd.assembler import Plan
-    mesh, M, dh, space, asm, mi = cfg1
+    mesh, M, dh, space, asm, mi = core20
     nu = torch.from_numpy(core_box_nu(mesh)).cuda()
     q = asm.stiffness_quadrature()
     pr = L.PRECISIONS[prec]
@@ -414,17 +414,17 @@ def test_csc_preloaded_coordinates(cfg1, prec, driver):
                                         L.PREC_FP64_EXACT, L.ptr(v2), L.stream_ptr()))
 
 
-def test_mass_weighted_empty_when_sigma_zero(cfg1):
+def test_mass_weighted_empty_when_sigma_zero(core20):
     """With sigma = 0 everywhere, sparse() of the weighted mass is empty."""
-    mesh, M, dh, space, asm, mi = cfg1
+    mesh, M, dh, space, asm, mi = core20
     Mg = asm.assembleMassWeighted(space, np.zeros(mesh.num_elems), precision="fp64_exact")
     assert Mg.nnz == 0
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
-def test_csc_frequency_operator(cfg1, prec, driver):
+def test_csc_frequency_operator(core20, prec, driver):
     _one_exact_driver(prec, driver)
-    mesh, M, dh, space, asm, mi = cfg1
+    mesh, M, dh, space, asm, mi = core20
     nu = core_box_nu(mesh)
     sig = np.where(mesh.RegionTags == 2, 3.5e7, 0.0)
     omega = 2 * np.pi * 50.0
@@ -447,9 +447,9 @@ def test_csc_frequency_operator(cfg1, prec, driver):
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_local"])
-def test_csc_jacobian(cfg1, prec, driver):
+def test_csc_jacobian(core20, prec, driver):
     _one_exact_driver(prec, driver)
-    mesh, M, dh, space, asm, mi = cfg1
+    mesh, M, dh, space, asm, mi = core20
     lib = {1: create_linear(1.0), 2: marrocco()}
     packed = pack_material_data(lib)
     qw, curl, _ = stiffness_tables()
@@ -481,11 +481,11 @@ def test_csc_jacobian(cfg1, prec, driver):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_local"])
-def test_csc_newton_set(cfg1, prec, driver):
+def test_csc_newton_set(core20, prec, driver):
     """Fused K(nu(B)) + M_sigma + J + R against the oracle's separate kernels:
     J and R from the Jacobian kernel, M from the mass kernel, K from the
     curl-curl kernel with nu_e = eval_nu(|B_e|^2) at the element's B."""
-    mesh, M, dh, space, asm, mi = cfg1
+    mesh, M, dh, space, asm, mi = core20
     lib = {1: create_linear(1.0), 2: marrocco()}
     packed = pack_material_data(lib)
     qw, curl, _ = stiffness_tables()
