@@ -368,7 +368,7 @@ typedef struct mpk_batch mpk_batch;
 int mpk_batch_create(const mpk_batch_item* items, int64_t count, const double* omegas,
                      int32_t nfreq, const mpk_quadrature* quad_k, const mpk_quadrature* quad_m,
                      int precision, void* stream, mpk_batch** batch_out);
-/* Schedule: ring gather batch (2..4) and blocks per CU of a persistent grid
+/* Schedule: ring gather batch (2, 3, 4 or 6) and blocks per CU of a persistent grid
  * (0: one block per 64-column tile).  Results do not depend on it. */
 int mpk_batch_schedule(mpk_batch* batch, int32_t ring_batch, int32_t blocks_per_cu);
 int mpk_batch_info(const mpk_batch* batch, int64_t* ntiles, int64_t* nnz_total,

@@ -26,7 +26,7 @@ from . import _lib as L
 from .assembler import Assembler, DofHandler, FunctionSpace, Mesh
 
 # schedules the autotuner tries: (ring gather batch, blocks per CU; 0 = one block per tile)
-SCHEDULES = tuple((kb, bpc) for kb in (2, 3, 4) for bpc in (0, 8, 16))
+SCHEDULES = tuple((kb, bpc) for kb in (2, 3, 4, 6) for bpc in (0, 8, 16))
 
 
 class BatchAssembler:

@@ -164,8 +164,8 @@ int mpk_batch_create(const mpk_batch_item* items, int64_t count, const double* o
 int mpk_batch_schedule(mpk_batch* b, int32_t ring_batch, int32_t blocks_per_cu) {
     clear_error();
     MPK_REQUIRE(b, MPK_ERR_ARGS, "NULL batch");
-    MPK_REQUIRE(ring_batch >= 2 && ring_batch <= 4, MPK_ERR_ARGS, "ring_batch %d outside [2, 4]",
-                ring_batch);
+    MPK_REQUIRE(ring_batch == 2 || ring_batch == 3 || ring_batch == 4 || ring_batch == 6,
+                MPK_ERR_ARGS, "ring_batch %d not one of 2, 3, 4, 6", ring_batch);
     MPK_REQUIRE(blocks_per_cu >= 0 && blocks_per_cu <= 64, MPK_ERR_ARGS,
                 "blocks_per_cu %d outside [0, 64]", blocks_per_cu);
     b->ring_batch = ring_batch;
@@ -203,10 +203,12 @@ int mpk_batch_run(const mpk_batch* b, void* stream) {
     if (b->precision == MPK_PREC_FP64) {
         if (b->ring_batch == 2) MPK_BATCH_LAUNCH(MPK_PREC_FP64, 2);
         else if (b->ring_batch == 4) MPK_BATCH_LAUNCH(MPK_PREC_FP64, 4);
+        else if (b->ring_batch == 6) MPK_BATCH_LAUNCH(MPK_PREC_FP64, 6);
         else MPK_BATCH_LAUNCH(MPK_PREC_FP64, 3);
     } else {
         if (b->ring_batch == 2) MPK_BATCH_LAUNCH(MPK_PREC_FP32_LOCAL, 2);
         else if (b->ring_batch == 4) MPK_BATCH_LAUNCH(MPK_PREC_FP32_LOCAL, 4);
+        else if (b->ring_batch == 6) MPK_BATCH_LAUNCH(MPK_PREC_FP32_LOCAL, 6);
         else MPK_BATCH_LAUNCH(MPK_PREC_FP32_LOCAL, 3);
     }
 #undef MPK_BATCH_LAUNCH

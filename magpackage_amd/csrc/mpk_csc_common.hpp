@@ -150,11 +150,22 @@ __device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* 
     return tile_begin_at(tc, op, acc, smem, xcd_tile(blockIdx.x, gridDim.x));
 }
 
+// Ops may store a finished tile themselves (store_tile(base, acc, size)),
+// e.g. to order the stores of several output arrays.
+template <class Op, class = void>
+struct HasStoreTile : std::false_type {};
+template <class Op>
+struct HasStoreTile<Op, std::void_t<decltype(&Op::store_tile)>> : std::true_type {};
+
 // Tile epilogue: the finished accumulator range leaves as contiguous stores.
 template <class Op, class Acc>
 __device__ __forceinline__ void tile_end(const Op& op, const TileRange& t, const Acc* acc) {
     if constexpr (Planes<Op>::value == 0) return;
     __syncthreads();
+    if constexpr (HasStoreTile<Op>::value) {
+        op.store_tile(t.base, acc, t.size);
+        return;
+    }
     for (int i = threadIdx.x; i < t.size; i += kW) {
         if constexpr (Planes<Op>::value > 1)
             op.store_planes(t.base + i, acc + i, t.size);

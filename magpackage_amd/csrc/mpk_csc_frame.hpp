@@ -375,6 +375,15 @@ struct FrameNewton {
         if (kWithM) out_m[p] = a[size];
         if (out_j) out_j[p] = k + a[kD * size];
     }
+    // array-outer: each output's tile range is written as one contiguous run
+    __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
+        if (out_k)
+            for (int i = threadIdx.x; i < size; i += kW) out_k[base + i] = acc[i];
+        if (kWithM)
+            for (int i = threadIdx.x; i < size; i += kW) out_m[base + i] = acc[size + i];
+        if (out_j)
+            for (int i = threadIdx.x; i < size; i += kW) out_j[base + i] = acc[i] + acc[kD * size + i];
+    }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
     __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
@@ -450,6 +459,15 @@ struct FrameNewtonLin {
         if (out_k) out_k[p] = k;
         if (out_m) out_m[p] = 0.0;
         if (out_j) out_j[p] = k + 0.0;
+    }
+    // array-outer: each output's tile range is written as one contiguous run
+    __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
+        if (out_k)
+            for (int i = threadIdx.x; i < size; i += kW) out_k[base + i] = acc[i];
+        if (out_m)
+            for (int i = threadIdx.x; i < size; i += kW) out_m[base + i] = 0.0;
+        if (out_j)
+            for (int i = threadIdx.x; i < size; i += kW) out_j[base + i] = acc[i] + 0.0;
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load_bnu(e); }
@@ -688,6 +706,18 @@ struct FrameKMF {
     __device__ void col_end(int64_t, Col&) const {}
     __device__ void store(int64_t p, Acc a) const {
         for (int f = 0; f < nfreq; ++f) km.out[f * nnz + p] = make_double2(a.x, omegas[f] * a.y);
+    }
+    // frequency-outer: each value set's tile range is written as one
+    // contiguous run before the next set's
+    __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
+        for (int f = 0; f < nfreq; ++f) {
+            const double w = omegas[f];
+            double2* __restrict__ o = km.out + f * nnz + base;
+            for (int i = threadIdx.x; i < size; i += kW) {
+                const Acc a = acc[i];
+                o[i] = make_double2(a.x, w * a.y);
+            }
+        }
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids& d) const { return km.elem(e, typename FrameKM<kPrec>::Ids{}); }
