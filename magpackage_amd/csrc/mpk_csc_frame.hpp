@@ -572,11 +572,41 @@ __global__ void __launch_bounds__(kW) k_csc_frame(TileCtx tc, Op op) {
 // in flight per level instead of one.  Load indices are clamped to the last
 // incidence (always valid addresses, no branches in the load stream); the
 // arithmetic of slots past the column's count is skipped.
+// Every index of a tile depends on the tile and the lane only: the loads go
+// out together, one memory latency before the records instead of a chain of
+// dependent ones.
+struct RingIdx {
+    int cbase, inc0, inc1;
+    int2 lh;
+    int base, top, fbase;
+    int32_t cls;
+};
+__device__ __forceinline__ RingIdx ring_load(const TileCtx& tc, const int2* __restrict__ ends,
+                                             int64_t tile_id, const int32_t* __restrict__ sel,
+                                             int32_t want) {
+    const int64_t j0 = tile_id * kW;
+    const int64_t jend = j0 + kW < tc.n ? j0 + kW : tc.n;
+    const int64_t j = j0 + threadIdx.x;
+    const int64_t jc = j < jend ? j : jend - 1;  // a valid column for every lane
+    RingIdx ix;
+    ix.cbase = tc.colptr[jc];
+    ix.inc0 = tc.inc_ptr[jc];
+    ix.inc1 = tc.inc_ptr[jc + 1];
+    ix.lh = ends[jc];
+    ix.base = tc.colptr[j0];
+    ix.top = tc.colptr[jend];
+    ix.fbase = tc.frame_base[tile_id];
+    ix.cls = sel ? sel[tile_id] : want;
+    return ix;
+}
+
+// One tile of the ring driver from its loaded indices.  A tile of another
+// class is not branched around (the compiler would hoist that branch above
+// the index loads): it runs with no live lanes and an empty range, so it
+// writes nothing.
 template <class Op, int kB>
-__device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restrict__ ends, Op& op,
-                                          unsigned char* smem, int64_t tile_id,
-                                          const int32_t* __restrict__ sel = nullptr,
-                                          int32_t want = 0) {
+__device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned char* smem,
+                                         int64_t tile_id, const RingIdx& ix, int32_t want) {
     using Acc = typename Op::Acc;
     using Ids = typename Op::Ids;
     using Elem = typename Op::Elem;
@@ -587,21 +617,10 @@ __device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restr
     tr.j0 = tile_id * kW;
     tr.jend = tr.j0 + kW < tc.n ? tr.j0 + kW : tc.n;
     const int64_t j = tr.j0 + lane;
-    const int64_t jc = j < tr.jend ? j : tr.jend - 1;  // a valid column for every lane
-    // Every index of the tile depends on the tile and the lane only: the
-    // loads go out together, one memory latency before the records instead of
-    // a chain of dependent ones.  A tile of another class is not branched
-    // around (the compiler would hoist that branch above the loads): it runs
-    // with no live lanes and an empty range, so it writes nothing.
-    const int cbase = tc.colptr[jc];
-    const int inc0 = tc.inc_ptr[jc];
-    const int inc1 = tc.inc_ptr[jc + 1];
-    const int2 lh = ends[jc];
-    const int base = tc.colptr[tr.j0];
-    const int top = tc.colptr[tr.jend];
-    const int fbase = tc.frame_base[tile_id];
-    const int32_t cls = sel ? sel[tile_id] : want;
-    const bool mine = cls == want;
+    const int cbase = ix.cbase, inc0 = ix.inc0, inc1 = ix.inc1;
+    const int2 lh = ix.lh;
+    const int base = ix.base, top = ix.top, fbase = ix.fbase;
+    const bool mine = ix.cls == want;
     const bool live = mine && j < tr.jend;
     tr.base = base;
     tr.size = mine ? top - base : 0;
@@ -654,6 +673,15 @@ __device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restr
         op.col_end(j, cs);
     }
     tile_end(op, tr, acc);
+}
+
+template <class Op, int kB>
+__device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restrict__ ends, Op& op,
+                                          unsigned char* smem, int64_t tile_id,
+                                          const int32_t* __restrict__ sel = nullptr,
+                                          int32_t want = 0) {
+    const RingIdx ix = ring_load(tc, ends, tile_id, sel, want);
+    ring_run<Op, kB>(tc, op, smem, tile_id, ix, want);
 }
 
 // sel (optional): per-tile class flags; a block whose tile's flag differs
