@@ -93,24 +93,34 @@ def cpu_baseline_newton(sample: int, reps: int):
     tags = km.RegionTags.astype(np.float64)
 
     def run():
+        ts = 0.0
         t0 = time.perf_counter()
         I, J, V, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, a, qw,
                                      curl, packed)
+        u = time.perf_counter()
         Jm = O.sparse_scipy(I, J, V, n, n)
         R = np.zeros(n)
         np.add.at(R, Ri.astype(np.int64) - 1, Rv)
+        ts += time.perf_counter() - u
         B = O.element_b(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a, curl)
         nu, _ = O.eval_nu((B * B).sum(axis=1), km.RegionTags, packed)
-        K = O.sparse_scipy(*O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl),
-                           n, n)
-        M = O.sparse_scipy(*O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sigma, qw, val),
-                           n, n)
+        trip = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl)
+        u = time.perf_counter()
+        K = O.sparse_scipy(*trip, n, n)
+        ts += time.perf_counter() - u
+        trip = O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sigma, qw, val)
+        u = time.perf_counter()
+        M = O.sparse_scipy(*trip, n, n)
+        ts += time.perf_counter() - u
+        del trip
         t1 = time.perf_counter()
-        return t1 - t0, (Jm.nnz, K.nnz, M.nnz, R.size)
+        return t1 - t0, (t1 - t0) - ts, ts
 
     run()
-    times = [run()[0] for _ in range(reps)]
-    t = float(np.median(times))
+    runs = [run() for _ in range(reps)]
+    t = float(np.median([r[0] for r in runs]))
+    tk = float(np.median([r[1] for r in runs]))
+    tsp = float(np.median([r[2] for r in runs]))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": km.num_elems / t,
@@ -119,7 +129,8 @@ def cpu_baseline_newton(sample: int, reps: int):
         "kind": "port",
         "sample": (f"Kuhn {sample}x{sample}x{nz} box ({km.num_elems} tets, same core and "
                    f"curve), Jacobian + B + curl-curl + mass kernels on {cores} OpenMP threads "
-                   f"+ serial sparse() of K, M, J, median of {reps} after 1 warm-up: {t:.3f} s"),
+                   f"+ serial sparse() of K, M, J, median of {reps} after 1 warm-up: {t:.3f} s "
+                   f"(kernels {tk:.3f} s, sparse() {tsp:.3f} s)"),
     }
 
 
@@ -486,18 +497,23 @@ def cpu_baseline(sample_n: int, reps: int):
     def run():
         t0 = time.perf_counter()
         I, J, V = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl)
+        t1 = time.perf_counter()
         K = O.sparse_scipy(I, J, V, n, n)
         del I, J, V
+        t2 = time.perf_counter()
         I, J, V = O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sigma, qw, val)
+        t3 = time.perf_counter()
         M = O.sparse_scipy(I, J, V, n, n)
         del I, J, V
         A = K + 1j * omega * M
-        t1 = time.perf_counter()
-        return t1 - t0, A.nnz
+        t4 = time.perf_counter()
+        return t4 - t0, (t1 - t0) + (t3 - t2), (t2 - t1) + (t4 - t3)
 
     run()  # warm-up
-    times = [run()[0] for _ in range(reps)]
-    t = float(np.median(times))
+    runs = [run() for _ in range(reps)]
+    t = float(np.median([r[0] for r in runs]))
+    tk = float(np.median([r[1] for r in runs]))
+    ts = float(np.median([r[2] for r in runs]))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": km.num_elems / t,
@@ -506,7 +522,8 @@ def cpu_baseline(sample_n: int, reps: int):
         "kind": "port",
         "sample": (f"Kuhn {sample_n}^3 box ({km.num_elems} tets, same plate/sigma/50 Hz), "
                    f"COO kernels on {cores} OpenMP threads + serial sparse() + K+1j*w*M, "
-                   f"median of {reps} after 1 warm-up: {t:.3f} s"),
+                   f"median of {reps} after 1 warm-up: {t:.3f} s (COO kernels {tk:.3f} s, "
+                   f"sparse() and K+1j*w*M {ts:.3f} s)"),
     }
 
 
@@ -554,7 +571,7 @@ def main():
     ap.add_argument("--precision", default=None, choices=["fp64", "fp64_exact", "fp32_local"],
                     help="default fp64 (freq128) / fp32_local (newton4m)")
     ap.add_argument("--cpu-sample", type=int, default=80)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="after timing, run a plain 512 MiB elementwise copy (reads and writes "
