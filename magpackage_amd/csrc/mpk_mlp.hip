@@ -60,9 +60,20 @@ __device__ __forceinline__ bf4 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_
     v.w = (short)d;
     return v;
 }
-// 4 consecutive BF16 of a row-major LDS tile
+// 4 consecutive BF16 of a row-major LDS tile, one 8-byte read (idx is a
+// multiple of 4 at every call and the tiles are 8-byte aligned)
 __device__ __forceinline__ bf4 ld_row4(const uint16_t* t, int idx) {
-    return pack4(t[idx], t[idx + 1], t[idx + 2], t[idx + 3]);
+    bf4 v;
+    __builtin_memcpy(&v, t + idx, sizeof(v));
+    return v;
+}
+// Orders the LDS accesses of one wave: the activation tiles inside the chunk
+// loops belong to one wave, whose LDS instructions execute in order, so a
+// fence that keeps the compiler from moving them replaces a block barrier.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 // 4 BF16 of one column, rows r..r+3, row stride ld
 __device__ __forceinline__ bf4 ld_col4(const uint16_t* t, int idx, int ld) {
@@ -73,7 +84,7 @@ __device__ __forceinline__ f4 mfma(bf4 a, bf4 b, f4 c) {
 }
 __device__ __forceinline__ f4 zero4() { return f4{0.f, 0.f, 0.f, 0.f}; }
 
-struct Smem {
+struct alignas(16) Smem {
     uint16_t w1[kH * kIn];
     uint16_t w2[kH * kH];
     uint16_t w3[kH * kH];
@@ -102,8 +113,7 @@ __device__ void load_weights(Smem& s, const float* __restrict__ p, const uint16_
 // Forward pass of one 16-row chunk (rows r0.., n valid in total).  Leaves
 // BF16 activations in LDS, returns the BF16-rounded layer-3 activations of
 // the lane's D entries (hb3[ob][i]: row 4g+i, unit 16 ob + c) and y of rows
-// 4g+i (identical in all 16 lanes of a group).  Block-uniform: every wave of
-// the block calls it the same number of times (barriers inside).
+// 4g+i (identical in all 16 lanes of a group).  Touches only wave w's tiles.
 __device__ void forward_chunk(Smem& s, int w, const float* __restrict__ X, int64_t r0, int64_t n,
                               float b4, float (&hb3)[4][4], float (&y)[4]) {
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
@@ -116,7 +126,7 @@ __device__ void forward_chunk(Smem& s, int w, const float* __restrict__ X, int64
         const float v = (r0 + r < n) ? X[(r0 + r) * kIn + k] : 0.0f;
         xs[e] = f2bf(v);
     }
-    __syncthreads();
+    wave_sync();
     // layer 1: Z1 = X W1^T + b1
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
@@ -126,7 +136,7 @@ __device__ void forward_chunk(Smem& s, int w, const float* __restrict__ X, int64
 #pragma unroll
         for (int i = 0; i < 4; ++i) h1[(4 * g + i) * kH + o] = f2bf(tanhf(acc[i] + s.b1[o]));
     }
-    __syncthreads();
+    wave_sync();
     // layer 2
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
@@ -139,7 +149,7 @@ __device__ void forward_chunk(Smem& s, int w, const float* __restrict__ X, int64
 #pragma unroll
         for (int i = 0; i < 4; ++i) h2[(4 * g + i) * kH + o] = f2bf(tanhf(acc[i] + s.b2[o]));
     }
-    __syncthreads();
+    wave_sync();
     // layer 3 and the linear output
     float part[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -167,7 +177,7 @@ __device__ void forward_chunk(Smem& s, int w, const float* __restrict__ X, int64
         v += __shfl_xor(v, 1);
         y[i] = v + b4;
     }
-    __syncthreads();
+    wave_sync();
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -255,7 +265,7 @@ k_mlp_grad(const float* __restrict__ p, const uint16_t* __restrict__ wb,
                 dz[(4 * g + i) * kH + o] = f2bf(z);
             }
         }
-        __syncthreads();
+        wave_sync();
         // gW3 += dZ3^T H2;  dH2 = dZ3 W3
         float dh[4][4];
 #pragma unroll
@@ -277,7 +287,7 @@ k_mlp_grad(const float* __restrict__ p, const uint16_t* __restrict__ wb,
                 dh[kb][i] = acc[i] * (1.0f - hb * hb);
             }
         }
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
@@ -285,7 +295,7 @@ k_mlp_grad(const float* __restrict__ p, const uint16_t* __restrict__ wb,
                 gb2[kb] += dh[kb][i];
                 dz[(4 * g + i) * kH + kb * 16 + c] = f2bf(dh[kb][i]);
             }
-        __syncthreads();
+        wave_sync();
         // gW2 += dZ2^T H1;  dH1 = dZ2 W2
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob)
@@ -306,7 +316,7 @@ k_mlp_grad(const float* __restrict__ p, const uint16_t* __restrict__ wb,
                 dh[kb][i] = acc[i] * (1.0f - hb * hb);
             }
         }
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
@@ -314,13 +324,13 @@ k_mlp_grad(const float* __restrict__ p, const uint16_t* __restrict__ wb,
                 gb1[kb] += dh[kb][i];
                 dz[(4 * g + i) * kH + kb * 16 + c] = f2bf(dh[kb][i]);
             }
-        __syncthreads();
+        wave_sync();
         // gW1 += dZ1^T X
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob)
             gW1[ob] = mfma(ld_col4(dz, (4 * g) * kH + ob * 16 + c, kH),
                            ld_col4(xs, (4 * g) * kIn + c, kIn), gW1[ob]);
-        __syncthreads();
+        wave_sync();
     }
     // partial gradient of this wave
     float* out = work + (int64_t)(blockIdx.x * kWaves + w) * kStride;
