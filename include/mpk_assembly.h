@@ -411,6 +411,27 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
                       double* x, double* work, double rel_tol, int max_iter, int check_every,
                       int* iters_out, double* rel_res_out, void* stream);
 
+/* Doubles of device workspace mpk_cocg_aux_solve needs. */
+int mpk_cocg_aux_work_size(int64_t n, int64_t num_nodes, int64_t* size_out);
+
+/* Complex symmetric solve for the time-harmonic operator the reference hands
+ * to MUMPS in symmetric mode (src/solver/FrequencySolver.m:127-139,
+ * LinearSolver with MumpsSymmetry = 2): A = R + 1j*I, R and I real symmetric
+ * on one structural CSC pattern (vals_re, vals_im), e.g. R = K + eps*M and
+ * I = omega*M_sigma.  Dirichlet DOFs as in mpk_pcg_solve.  COCG (CG with the
+ * unconjugated form x^T y) preconditioned by the real operator of
+ * mpk_pcg_aux_solve built from P = R + I: Jacobi on diag(R + I) plus the
+ * gradient-space correction (aux: Ln = G^T (mass part of P) G; NULL = Jacobi
+ * only), applied to the real and imaginary parts.  b and x are complex128
+ * [n] (interleaved re, im; 16-byte aligned), work 16-byte aligned.  x starts
+ * at 0; stops when |r|_2/|b|_2 <= rel_tol (checked every check_every
+ * iterations) or after max_iter.  Deterministic; synchronises. */
+int mpk_cocg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double* vals_re,
+                       const double* vals_im, int64_t n, const uint8_t* fixed,
+                       const mpk_aux_space* aux, const double* b, double* x, double* work,
+                       double rel_tol, int max_iter, int check_every, int* iters_out,
+                       double* rel_res_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

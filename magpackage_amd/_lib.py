@@ -125,6 +125,10 @@ _SIGS = {
     "mpk_batch_destroy": ([_vp], _i32),
     "mpk_pcg_aux_solve": ([_vp, _vp, _vp, _i64, _vp, C.POINTER(AuxSpace), _vp, _vp, _vp, _d, _i32,
                            _i32, C.POINTER(C.c_int), C.POINTER(C.c_double), _vp], _i32),
+    "mpk_cocg_aux_work_size": ([_i64, _i64, _pi64], _i32),
+    "mpk_cocg_aux_solve": ([_vp, _vp, _vp, _vp, _i64, _vp, C.POINTER(AuxSpace), _vp, _vp, _vp,
+                            _d, _i32, _i32, C.POINTER(C.c_int), C.POINTER(C.c_double), _vp],
+                           _i32),
     "mpk_mlp_sync_bf16": ([_vp, _vp, _vp], _i32),
     "mpk_mlp_forward": ([_vp, _vp, _vp, _i64, _vp, _vp], _i32),
     "mpk_mlp_work_size": ([_i64, _pi64], _i32),

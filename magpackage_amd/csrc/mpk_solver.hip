@@ -356,6 +356,271 @@ __global__ void __launch_bounds__(kB) k_copy(int64_t n, const double* __restrict
         b[i] = a[i];
 }
 
+// ---- complex symmetric systems (COCG) -----------------------------------------
+// A = R + iI with R, I real symmetric on one structural CSC pattern; vectors are
+// complex128 (interleaved re, im).  COCG (van der Vorst and Melissen, 1990) is
+// CG with the unconjugated bilinear form x^T y, valid for complex symmetric A
+// and a complex symmetric preconditioner.  The preconditioner is the real
+// auxiliary-space operator above, built from P = R + I (SPD when R is SPD and
+// I is positive semi-definite), applied to the real and imaginary parts alike.
+// For R, I >= 0 the eigenvalues of P^-1 A lie on {(1 + i t)/(1 + t), t >= 0},
+// modulus in [1/sqrt(2), 1], so the iteration count follows that of P.
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cdiv(double2 a, double2 b) {
+    const double d = b.x * b.x + b.y * b.y;
+    if (d == 0.0) return make_double2(0.0, 0.0);
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// Writes a block's (re, im) partial sums: wave butterflies, then wave order.
+__device__ inline void block_partial2(double a, double b, double* __restrict__ pa,
+                                      double* __restrict__ pb) {
+    __shared__ double red[2][kB / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0, t = 0.0;
+        for (int w = 0; w < kB / 64; ++w) {
+            s += red[0][w];
+            t += red[1][w];
+        }
+        pa[blockIdx.x] = s;
+        if (pb) pb[blockIdx.x] = t;
+    }
+}
+
+// q = A_bc p (fixed rows and columns removed, unit diagonal) and partials of p^T q.
+__global__ void __launch_bounds__(kB)
+k_c_spmv_dot(const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+             const double* __restrict__ vr, const double* __restrict__ vi,
+             const uint8_t* __restrict__ fixed, int64_t n, const double2* __restrict__ x,
+             double2* __restrict__ y, double* __restrict__ pre, double* __restrict__ pim) {
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / kRowLanes);
+    double dre = 0.0, dim = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * (kB / kRowLanes) + threadIdx.x / kRowLanes; i < n;
+         i += rows_per_grid) {
+        double sr = 0.0, si = 0.0;
+        const bool fi = fixed && fixed[i];
+        if (!fi) {
+            for (int k = colptr[i] + lane; k < colptr[i + 1]; k += kRowLanes) {
+                const int j = rowind[k];
+                if (!(fixed && fixed[j])) {
+                    const double2 xj = x[j];
+                    const double a = vr[k], b = vi[k];
+                    sr += a * xj.x - b * xj.y;
+                    si += a * xj.y + b * xj.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, kRowLanes);
+            si += __shfl_xor(si, off, kRowLanes);
+        }
+        if (lane == 0) {
+            const double2 xi = x[i];
+            if (fi) {
+                sr = xi.x;
+                si = xi.y;
+            }
+            y[i] = make_double2(sr, si);
+            dre += xi.x * sr - xi.y * si;
+            dim += xi.x * si + xi.y * sr;
+        }
+    }
+    block_partial2(dre, dim, pre, pim);
+}
+
+// dinv = 1/diag(R + I) (1 on fixed rows), r = b (0 on fixed rows), x = 0, partials of |r|^2.
+__global__ void __launch_bounds__(kB)
+k_c_setup(const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+          const double* __restrict__ vr, const double* __restrict__ vi,
+          const uint8_t* __restrict__ fixed, int64_t n, const double2* __restrict__ b,
+          double* __restrict__ dinv, double2* __restrict__ r, double2* __restrict__ x,
+          double* __restrict__ prr) {
+    double rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+        const bool fi = fixed && fixed[i];
+        double d = 1.0;
+        if (!fi) {
+            d = 0.0;
+            for (int k = colptr[i]; k < colptr[i + 1]; ++k)
+                if (rowind[k] == i) d += vr[k] + vi[k];
+        }
+        dinv[i] = (d != 0.0) ? 1.0 / d : 1.0;
+        const double2 ri = fi ? make_double2(0.0, 0.0) : b[i];
+        r[i] = ri;
+        x[i] = make_double2(0.0, 0.0);
+        rr += ri.x * ri.x + ri.y * ri.y;
+    }
+    block_partial2(rr, 0.0, prr, nullptr);
+}
+
+__global__ void __launch_bounds__(kB)
+k_c_gt(int64_t nv, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nref,
+       const double2* __restrict__ r, double2* __restrict__ g) {
+    for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kB) {
+        double sr = 0.0, si = 0.0;
+        for (int k = nptr[v]; k < nptr[v + 1]; ++k) {
+            const int ref = nref[k];
+            const double2 rv = r[ref >> 1];
+            if (ref & 1) {
+                sr += rv.x;
+                si += rv.y;
+            } else {
+                sr -= rv.x;
+                si -= rv.y;
+            }
+        }
+        g[v] = make_double2(sr, si);
+    }
+}
+
+__global__ void __launch_bounds__(kB)
+k_c_jacobi_first(int64_t nv, const double* __restrict__ dninv, const double2* __restrict__ g,
+                 double omega, double2* __restrict__ phi) {
+    for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kB) {
+        const double s = omega * dninv[v];
+        const double2 gv = g[v];
+        phi[v] = make_double2(s * gv.x, s * gv.y);
+    }
+}
+
+__global__ void __launch_bounds__(kB)
+k_c_jacobi_sweep(const int32_t* __restrict__ lp, const int32_t* __restrict__ li,
+                 const double* __restrict__ lv, int64_t nv, const double* __restrict__ dninv,
+                 const double2* __restrict__ g, double omega, const double2* __restrict__ phi_in,
+                 double2* __restrict__ phi_out) {
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / kRowLanes);
+    for (int64_t v = (int64_t)blockIdx.x * (kB / kRowLanes) + threadIdx.x / kRowLanes; v < nv;
+         v += rows_per_grid) {
+        double sr = 0.0, si = 0.0;
+        for (int k = lp[v] + lane; k < lp[v + 1]; k += kRowLanes) {
+            const double2 pj = phi_in[li[k]];
+            sr += lv[k] * pj.x;
+            si += lv[k] * pj.y;
+        }
+#pragma unroll
+        for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, kRowLanes);
+            si += __shfl_xor(si, off, kRowLanes);
+        }
+        if (lane == 0) {
+            const double s = omega * dninv[v];
+            const double2 pv = phi_in[v], gv = g[v];
+            phi_out[v] = make_double2(pv.x + s * (gv.x - sr), pv.y + s * (gv.y - si));
+        }
+    }
+}
+
+// z = D^-1 r + G phi (0 on fixed rows) and partials of r^T z.
+__global__ void __launch_bounds__(kB)
+k_c_z(int64_t n, const int2* __restrict__ en, const uint8_t* __restrict__ fixed,
+      const double* __restrict__ dinv, const double2* __restrict__ r,
+      const double2* __restrict__ phi, double2* __restrict__ z, double* __restrict__ pre,
+      double* __restrict__ pim) {
+    double dre = 0.0, dim = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+        double2 zi = make_double2(0.0, 0.0);
+        const double2 ri = r[i];
+        if (!(fixed && fixed[i])) {
+            zi = make_double2(dinv[i] * ri.x, dinv[i] * ri.y);
+            if (phi) {
+                const int2 ab = en[i];
+                if (ab.y >= 0) {
+                    zi.x += phi[ab.y].x;
+                    zi.y += phi[ab.y].y;
+                }
+                if (ab.x >= 0) {
+                    zi.x -= phi[ab.x].x;
+                    zi.y -= phi[ab.x].y;
+                }
+            }
+        }
+        z[i] = zi;
+        dre += ri.x * zi.x - ri.y * zi.y;
+        dim += ri.x * zi.y + ri.y * zi.x;
+    }
+    block_partial2(dre, dim, pre, pim);
+}
+
+// scal: [0,1] rho, [2,3] alpha, [4,5] beta, [6] |r|^2
+__global__ void __launch_bounds__(kB) k_c_alpha(const double* __restrict__ pre,
+                                                const double* __restrict__ pim, int nparts,
+                                                double* __restrict__ scal) {
+    const double mr = block_sum_fixed(pre, nparts);
+    const double mi = block_sum_fixed(pim, nparts);
+    if (threadIdx.x == 0) {
+        const double2 a = cdiv(make_double2(scal[0], scal[1]), make_double2(mr, mi));
+        scal[2] = a.x;
+        scal[3] = a.y;
+    }
+}
+
+__global__ void __launch_bounds__(kB)
+k_c_update(int64_t n, const double* __restrict__ scal, const double2* __restrict__ p,
+           const double2* __restrict__ q, double2* __restrict__ x, double2* __restrict__ r,
+           double* __restrict__ prr) {
+    const double2 alpha = make_double2(scal[2], scal[3]);
+    double rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+        const double2 ap = cmul(alpha, p[i]);
+        const double2 aq = cmul(alpha, q[i]);
+        double2 xi = x[i];
+        xi.x += ap.x;
+        xi.y += ap.y;
+        x[i] = xi;
+        double2 ri = r[i];
+        ri.x -= aq.x;
+        ri.y -= aq.y;
+        r[i] = ri;
+        rr += ri.x * ri.x + ri.y * ri.y;
+    }
+    block_partial2(rr, 0.0, prr, nullptr);
+}
+
+// first == 1: rho = r^T z, p = z.  Otherwise beta = rho_new / rho.  Also |r|^2.
+__global__ void __launch_bounds__(kB)
+k_c_beta(const double* __restrict__ pre, const double* __restrict__ pim,
+         const double* __restrict__ prr, int nparts, int first, double* __restrict__ scal) {
+    const double zr = block_sum_fixed(pre, nparts);
+    const double zi = block_sum_fixed(pim, nparts);
+    const double rr = block_sum_fixed(prr, nparts);
+    if (threadIdx.x == 0) {
+        const double2 b = first ? make_double2(0.0, 0.0)
+                                : cdiv(make_double2(zr, zi), make_double2(scal[0], scal[1]));
+        scal[4] = b.x;
+        scal[5] = b.y;
+        scal[0] = zr;
+        scal[1] = zi;
+        scal[6] = rr;
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_c_direction(int64_t n, const double* __restrict__ scal,
+                                                    const double2* __restrict__ z,
+                                                    double2* __restrict__ p) {
+    const double2 beta = make_double2(scal[4], scal[5]);
+    for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+        const double2 bp = cmul(beta, p[i]);
+        const double2 zi = z[i];
+        p[i] = make_double2(zi.x + bp.x, zi.y + bp.y);
+    }
+}
+
 int grid_parts(int64_t n, int per_block) {
     int64_t g = (n + per_block - 1) / per_block;
     if (g > kMaxPartials) g = kMaxPartials;
@@ -549,6 +814,122 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
         MPK_HIP_TRY(hipMemcpyAsync(h, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
         MPK_HIP_TRY(hipStreamSynchronize(st));
         rel = std::sqrt(h[3]) / r0;
+        if (!(rel > rel_tol)) break;  // also stops on NaN
+    }
+    *iters_out = it;
+    *rel_res_out = rel;
+    return MPK_OK;
+}
+
+int mpk_cocg_aux_work_size(int64_t n, int64_t num_nodes, int64_t* size_out) {
+    clear_error();
+    MPK_REQUIRE(n >= 0 && num_nodes >= 0 && size_out, MPK_ERR_ARGS, "bad argument");
+    *size_out = 9 * (n + 1) + 7 * (num_nodes + 1) + 3 * (int64_t)kMaxPartials + 16;
+    return MPK_OK;
+}
+
+int mpk_cocg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double* vals_re,
+                       const double* vals_im, int64_t n, const uint8_t* fixed,
+                       const mpk_aux_space* aux, const double* b, double* x, double* work,
+                       double rel_tol, int max_iter, int check_every, int* iters_out,
+                       double* rel_res_out, void* stream) {
+    clear_error();
+    MPK_REQUIRE(n >= 0 && max_iter >= 0 && iters_out && rel_res_out, MPK_ERR_ARGS,
+                "bad argument");
+    *iters_out = 0;
+    *rel_res_out = 0.0;
+    if (n == 0) return MPK_OK;
+    MPK_REQUIRE(colptr && rowind && vals_re && vals_im && b && x && work, MPK_ERR_ARGS,
+                "NULL argument");
+    const int64_t nv = aux ? aux->num_nodes : 0;
+    MPK_REQUIRE(!aux || (nv >= 0 && aux->sweeps >= 1 && aux->edge_nodes &&
+                         (nv == 0 || (aux->node_ptr && aux->node_refs && aux->ln_ptr &&
+                                      aux->ln_idx && aux->ln_vals))),
+                MPK_ERR_ARGS, "incomplete auxiliary space");
+    if (check_every < 1) check_every = 1;
+    hipStream_t st = as_stream(stream);
+    // work: dinv (n, padded to even), r, z, p, q (2n each), dninv (nv, padded to even),
+    // g, phi0, phi1 (2nv each), partials (3 x kMaxPartials), scal (16)
+    double* dinv = work;
+    double2* r = reinterpret_cast<double2*>(dinv + ((n + 1) & ~(int64_t)1));
+    double2* z = r + n;
+    double2* p = z + n;
+    double2* q = p + n;
+    double* dninv = reinterpret_cast<double*>(q + n);
+    double2* g = reinterpret_cast<double2*>(dninv + ((nv + 1) & ~(int64_t)1));
+    double2* phi0 = g + nv;
+    double2* phi1 = phi0 + nv;
+    double* pa = reinterpret_cast<double*>(phi1 + nv);
+    double* pb = pa + kMaxPartials;
+    double* pc = pb + kMaxPartials;
+    double* scal = pc + kMaxPartials;
+    MPK_REQUIRE((reinterpret_cast<uintptr_t>(work) & 15) == 0, MPK_ERR_ARGS,
+                "work must be 16-byte aligned");
+    MPK_REQUIRE((reinterpret_cast<uintptr_t>(b) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(x) & 15) == 0,
+                MPK_ERR_ARGS, "b and x must be 16-byte aligned complex128 arrays");
+    const double2* b2 = reinterpret_cast<const double2*>(b);
+    double2* x2 = reinterpret_cast<double2*>(x);
+    const int g_vec = grid_parts(n, kB);
+    const int g_row = grid_parts(n, kB / kRowLanes);
+    const int g_nv = grid_parts(nv, kB);
+    const int g_nrow = grid_parts(nv, kB / kRowLanes);
+    const int2* en = aux ? reinterpret_cast<const int2*>(aux->edge_nodes) : nullptr;
+    // z = M^-1 r; partials of r^T z into (pa, pb)
+    auto precondition = [&]() {
+        const double2* phi = nullptr;
+        if (nv > 0) {
+            hipLaunchKernelGGL(k_c_gt, dim3(g_nv), dim3(kB), 0, st, nv, aux->node_ptr,
+                               aux->node_refs, r, g);
+            hipLaunchKernelGGL(k_c_jacobi_first, dim3(g_nv), dim3(kB), 0, st, nv, dninv, g,
+                               aux->omega, phi0);
+            double2* a = phi0;
+            double2* c = phi1;
+            for (int k = 1; k < aux->sweeps; ++k) {
+                hipLaunchKernelGGL(k_c_jacobi_sweep, dim3(g_nrow), dim3(kB), 0, st, aux->ln_ptr,
+                                   aux->ln_idx, aux->ln_vals, nv, dninv, g, aux->omega, a, c);
+                double2* t = a;
+                a = c;
+                c = t;
+            }
+            phi = a;
+        }
+        hipLaunchKernelGGL(k_c_z, dim3(g_vec), dim3(kB), 0, st, n, en, fixed, dinv, r, phi, z,
+                           pa, pb);
+    };
+    if (nv > 0)
+        hipLaunchKernelGGL(k_ln_dinv, dim3(g_nv), dim3(kB), 0, st, aux->ln_ptr, aux->ln_idx,
+                           aux->ln_vals, nv, dninv);
+    hipLaunchKernelGGL(k_c_setup, dim3(g_vec), dim3(kB), 0, st, colptr, rowind, vals_re, vals_im,
+                       fixed, n, b2, dinv, r, x2, pc);
+    precondition();
+    hipLaunchKernelGGL(k_c_beta, dim3(1), dim3(kB), 0, st, pa, pb, pc, g_vec, 1, scal);
+    hipLaunchKernelGGL(k_c_direction, dim3(g_vec), dim3(kB), 0, st, n, scal, z, p);
+    MPK_HIP_TRY(hipGetLastError());
+    double h[8];
+    MPK_HIP_TRY(hipMemcpyAsync(h, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MPK_HIP_TRY(hipStreamSynchronize(st));
+    const double r0 = std::sqrt(h[6]);
+    if (r0 == 0.0) return MPK_OK;
+    int it = 0;
+    double rel = 1.0;
+    while (it < max_iter) {
+        const int batch = (max_iter - it) < check_every ? (max_iter - it) : check_every;
+        for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL(k_c_spmv_dot, dim3(g_row), dim3(kB), 0, st, colptr, rowind,
+                               vals_re, vals_im, fixed, n, p, q, pa, pb);
+            hipLaunchKernelGGL(k_c_alpha, dim3(1), dim3(kB), 0, st, pa, pb, g_row, scal);
+            hipLaunchKernelGGL(k_c_update, dim3(g_vec), dim3(kB), 0, st, n, scal, p, q, x2, r,
+                               pc);
+            precondition();
+            hipLaunchKernelGGL(k_c_beta, dim3(1), dim3(kB), 0, st, pa, pb, pc, g_vec, 0, scal);
+            hipLaunchKernelGGL(k_c_direction, dim3(g_vec), dim3(kB), 0, st, n, scal, z, p);
+        }
+        MPK_HIP_TRY(hipGetLastError());
+        it += batch;
+        MPK_HIP_TRY(hipMemcpyAsync(h, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+        MPK_HIP_TRY(hipStreamSynchronize(st));
+        rel = std::sqrt(h[6]) / r0;
         if (!(rel > rel_tol)) break;  // also stops on NaN
     }
     *iters_out = it;

@@ -1,0 +1,111 @@
+"""Coil geometry helpers (src/utils/CoilGeometryUtils.m) on the CPU: the slice
+routine on a box whose cut is known exactly, the PCA normal, the direction
+fields' shape rules, and the TEAM Problem 7 coil of the reference's mesh
+against its published dimensions."""
+from pathlib import Path
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from magpackage_amd.coil import CoilGeometryUtils as CG
+from magpackage_amd.meshgen import kuhn_box
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _box(nx, ny, nz, lx, ly, lz, seed=3):
+    km = kuhn_box(nx, ny, nz, seed=seed)
+    P = km.P * np.array([lx, ly, lz])
+    return SimpleNamespace(P=P, T=km.T, RegionTags=np.full(km.T.shape[0], 7, dtype=np.int32),
+                           NumElements=km.T.shape[0])
+
+
+def test_slice_of_a_box_is_exact():
+    """A plane normal to x through the middle of a box (no node on it): the
+    cut is the full ly x lz rectangle, exactly.  Each cut polygon is placed
+    by the mean of its corners, which is its centroid only for triangles, so
+    the half-rectangles' centroids along y come out at +-ly/4 to within a
+    fraction of a cell (0.3 x 0.2 here)."""
+    m = _box(5, 4, 3, 2.0, 1.2, 0.6)
+    center = np.array([1.0, 0.6, 0.3])
+    area, cpos, cneg = CG.computeSliceProperties(m, 7, center, np.array([1.0, 0, 0]),
+                                                 np.array([0, 1.0, 0]))
+    assert abs(area - 1.2 * 0.6) <= 1e-12
+    assert abs(cpos - 0.3) <= 0.01 and abs(cneg + 0.3) <= 0.01
+    # a plane that misses the region
+    a0, p0, n0 = CG.computeSliceProperties(m, 7, np.array([5.0, 0, 0]), np.array([1.0, 0, 0]),
+                                           np.array([0, 1.0, 0]))
+    assert a0 == 0.0 and p0 is None and n0 is None
+
+
+def test_pca_normal_of_flat_and_tall_point_sets():
+    rng = np.random.default_rng(0)
+    th = rng.uniform(0, 2 * np.pi, 4000)
+    r = rng.uniform(0.8, 1.0, 4000)
+    flat = np.stack([r * np.cos(th), r * np.sin(th), rng.uniform(-0.05, 0.05, 4000)])
+    n, c = CG.fit_circle_normal_robust(flat + np.array([[1.0], [2.0], [3.0]]))
+    assert abs(abs(n[2]) - 1) < 1e-3 and n[2] > 0
+    assert np.allclose(c, [1, 2, 3], atol=0.05)
+    tall = np.stack([r * np.cos(th), rng.uniform(-5, 5, 4000), r * np.sin(th)])
+    n2, _ = CG.fit_circle_normal_robust(tall)
+    assert abs(abs(n2[1]) - 1) < 1e-3 and n2[1] > 0
+
+
+def test_direction_fields_shape_rules():
+    """Unit vectors inside the region, zero outside; the rounded rectangle's
+    sides run along U / V with the reference's signs, its corners are
+    tangent to the corner circles; all three fields circulate the same way
+    ((r x d) . n > 0 about a +z normal)."""
+    m = _box(8, 8, 2, 2.0, 2.0, 0.2)
+    m.RegionTags[:5] = 1
+    ctr = np.array([1.0, 1.0, 0.1])
+    cen = CG.getElementCenters(m)
+    inside = m.RegionTags == 7
+    for d in (CG.computeCircularDirection(m, 7, ctr, 3),
+              CG.computeRacetrackDirection(m, 7, ctr, 0.8, 3),
+              CG.computeRoundedRectDirection(m, 7, ctr, 0.8, 0.6, 3)):
+        assert d.shape == (3, m.NumElements)
+        assert np.all(d[:, ~inside] == 0)
+        nrm = np.linalg.norm(d[:, inside], axis=0)
+        assert np.allclose(nrm[nrm > 0], 1.0)
+        rel = cen[:, inside] - ctr[:, None]
+        circ = rel[0] * d[1, inside] - rel[1] * d[0, inside]
+        assert np.all(circ[nrm > 0] > 0)
+    d = CG.computeRoundedRectDirection(m, 7, ctr, 0.8, 0.6, 3)
+    u, v = cen[0] - 1.0, cen[1] - 1.0
+    tb = inside & (np.abs(u) <= 0.4)
+    assert np.array_equal(d[0, tb], -np.sign(v[tb])) and np.all(d[1, tb] == 0)
+    lr = inside & (np.abs(v) <= 0.3) & ~tb
+    assert np.array_equal(d[1, lr], np.sign(u[lr])) and np.all(d[0, lr] == 0)
+    co = inside & ~tb & ~lr
+    du, dv = u[co] - np.sign(u[co]) * 0.4, v[co] - np.sign(v[co]) * 0.3
+    assert np.allclose(d[0, co] * du + d[1, co] * dv, 0.0, atol=1e-12)
+
+
+def test_uv_indices():
+    assert CG.getUVIndices(1) == (2, 1)
+    assert CG.getUVIndices(2) == (0, 2)
+    assert CG.getUVIndices(3) == (0, 1)
+
+
+def test_team7_coil_dimensions():
+    """The TEAM Problem 7 coil (regions 3-6 of data/Team7.mphtxt, merged as
+    tutorials/run_team7.m does): published geometry 25 mm x 100 mm section,
+    skeleton 175 mm square with 37.5 mm corner radius (inner 25, outer 50),
+    centre (194, 100, 99) mm, axis z.  The detector measures it from the
+    tets within a few percent; the section area within 0.1%."""
+    d = np.load(GOLDEN / "team7_mesh.npz")
+    tags = d["tags"].copy()
+    tags[np.isin(tags, [3, 4, 5, 6])] = 9999
+    m = SimpleNamespace(P=d["P"] * 1e-3, T=d["T"], RegionTags=tags, NumElements=len(tags))
+    center, R_in, R_out, Lx, Ly, area, ax = CG.autoDetectRoundedRect(m, 9999)
+    assert ax == 3
+    assert np.allclose(center, [0.194, 0.100, 0.099], atol=1.5e-3)
+    assert abs(area - 2.5e-3) <= 2.5e-6
+    assert abs(Lx - 0.100) <= 5e-3 and abs(Ly - 0.100) <= 5e-3
+    assert abs(R_in - 0.025) <= 2e-3 and abs(R_out - 0.050) <= 2e-3
+    dm = CG.computeRoundedRectDirection(m, 9999, center, Lx, Ly, ax)
+    assert np.allclose(np.linalg.norm(dm[:, tags == 9999], axis=0), 1.0)
+    c2, R, a2, ax2 = CG.autoDetectCircular(m, 9999)
+    assert ax2 == 3 and abs(a2 - area) <= 1e-12
