@@ -541,6 +541,54 @@ class Assembler:
                                      L.ptr(F), L.stream_ptr()))
         return F[: plan.n]
 
+    def assembleWinding(self, space, windingObj) -> torch.Tensor:
+        """C = assembleWinding(space, windingObj) (src/Assembler.m:134-180):
+        assemble_winding_kernel_mex at the config's quadrature order with
+        scale Turns / CrossSectionArea, then sparse(I(valid), 1, V(valid), N, 1),
+        as a dense [N] device vector; a list of windings gives [N, k]."""
+        if isinstance(windingObj, (list, tuple)):
+            return torch.stack([self.assembleWinding(space, w) for w in windingObj], dim=1)
+        mesh = self.Mesh
+        cell = self.DofHandler.DofMaps[space.toString()]
+        n = self.DofHandler.NumGlobalDofs
+        ne, nn = mesh.NumElements, mesh.NumNodes
+        dev = _dev()
+        # MATLAB [k x Ne] column-major arrays are this package's (Ne, k) row-major ones
+        Td = mesh.T_dev.to(torch.float64)
+        Dd = cell.reshape(-1).to(torch.float64)
+        Sd = mesh.T2E_Sign.reshape(-1).to(torch.float64)
+        tags = mesh.Tags_dev.to(torch.float64)
+        dirf = None
+        if windingObj.DirectionField is not None and np.size(windingObj.DirectionField):
+            df = np.asarray(windingObj.DirectionField, dtype=np.float64)
+            if df.shape != (3, ne):
+                raise ValueError("DirectionField must be [3 x NumElements]")
+            dirf = torch.from_numpy(np.ascontiguousarray(df.T)).to(dev)
+        uni = np.zeros(3)
+        d = np.asarray(windingObj.Direction, dtype=np.float64).ravel()
+        uni[: min(3, d.size)] = d[:3]
+        scale = float(windingObj.Turns) / float(windingObj.CrossSectionArea)
+        q = L.standard_quadrature(self.Config.DefaultQuadratureOrder)
+        Io = torch.empty(max(6 * ne, 1), dtype=torch.float64, device=dev)
+        Vo = torch.empty_like(Io)
+        L.check(L.lib.mpk_winding_coo(
+            L.ptr(mesh.P_dev), nn, L.ptr(Td), ne, L.ptr(Dd), L.ptr(Sd), L.ptr(tags),
+            float(windingObj.RegionID), L.ptr(dirf), uni.ctypes.data_as(C.c_void_p), scale,
+            C.byref(q), L.ptr(Io), L.ptr(Vo), L.stream_ptr()))
+        nt = 6 * ne
+        Jo = torch.ones_like(Io)
+        colptr = torch.empty(2, dtype=torch.int64, device=dev)
+        rowind = torch.empty(max(nt, 1), dtype=torch.int32, device=dev)
+        vals = torch.empty(max(nt, 1), dtype=torch.float64, device=dev)
+        nnz = C.c_int64(0)
+        L.check(L.lib.mpk_sparse_coo_to_csc(L.ptr(Io), L.ptr(Jo), L.ptr(Vo), nt, n, 1, 1,
+                                            L.ptr(colptr), L.ptr(rowind), L.ptr(vals), nt,
+                                            C.byref(nnz), L.stream_ptr()))
+        k = int(nnz.value)
+        out = torch.zeros(max(n, 1), dtype=torch.float64, device=dev)
+        out[rowind[:k].to(torch.int64)] = vals[:k]
+        return out[:n]
+
     def computeElementB(self, solutionA, space=None):
         """PostProcessor.computeElementB (src/post/PostProcessor.m:64-126) for one
         solution column: (Ne, 3) device tensor of B at the elements."""
@@ -565,4 +613,5 @@ class Assembler:
     assemble_jacobian = assembleJacobian
     assemble_frequency_operator = assembleFrequencyOperator
     assemble_source = assembleSource
+    assemble_winding = assembleWinding
     compute_element_b = computeElementB

@@ -1,0 +1,129 @@
+"""Field-circuit coupled transients (magpackage_amd.circuit) on the GPU:
+- against tests/circuit_ref.py, the CPU restatement of the reference's BDF1 /
+  BDF2 solvers with their monolithic saddle-point Newton system (Coulomb-gauge
+  multiplier, winding current rows) solved by sparse LU;
+- against the reference's own recorded run of the E-core inrush
+  (tests/golden/ecore_circuit_runs.npz, from data/V-1500-R-8-N-3000-step-1e3.mat:
+  tutorials/run_ecore_sdc_compare.m's set-up on data/Ecore.mphtxt).  The CPU
+  restatement reproduces that recording's first steps to ~1e-11."""
+from pathlib import Path
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from tests.circuit_ref import circuit_bdf_cpu
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+# DW465-50 B-H data of tutorials/run_ecore_sdc_compare.m:45-50
+B_DATA = [0, 0.1, 0.2, 0.3001, 0.4, 0.5001, 0.6001, 0.7, 0.8, 0.9001, 1, 1.1001, 1.2001, 1.3002,
+          1.4, 1.4999, 1.5999, 1.6991, 1.7987, 1.8978, 1.98, 1.9925, 2.118164, 3.37480104]
+H_DATA = [0, 7.86, 13.99, 19.51, 24.58, 29.42, 34.07, 38.64, 43.12, 47.59, 52.1, 56.77, 61.67,
+          67.19, 74.03, 83.86, 1.00E+02, 1.38E+02, 2.95E+02, 1.01E+03, 2500, 12500, 112500,
+          1112500]
+
+
+def _engine(P, T, tags):
+    from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
+    from magpackage_amd.boundary import find_outer_boundary_dofs
+    mesh = Mesh(P, T, tags)
+    dh = DofHandler(mesh)
+    space = FunctionSpace("Nedelec", 1)
+    dh.distributeDofs(space)
+    asm = Assembler(mesh, dh)
+    fixed = find_outer_boundary_dofs(mesh, dh, space)
+    return asm, space, fixed
+
+
+@pytest.mark.parametrize("order", [2, 1])
+def test_circuit_solver_matches_saddle_point_restatement(gpu, order):
+    """Small E-core box (2,304 tets), one winding of 300 turns on a 120 V /
+    50 Hz source through 5 ohm into saturation: currents and probe |B| per
+    step equal the monolithic CPU solve (tight Newton tolerance, so both
+    converge to the discrete solution of every step)."""
+    from magpackage_amd.circuit import (Circuit, TransientBDF1Solver, TransientBDF2Solver,
+                                        Winding)
+    from magpackage_amd.material import create_linear, create_nonlinear
+    from magpackage_amd.meshgen import ecore_box
+    em = ecore_box(8, 8, 6)
+    km = em.mesh
+    lib = {1: create_linear(1.0), 2: create_nonlinear(B_DATA, H_DATA), 3: create_linear(1.0)}
+    V = lambda t: 120.0 * np.sin(2 * np.pi * 50 * t)     # noqa: E731
+    steps = [1e-3] * 6
+    probe = np.array([0.0503, 0.0811, 0.0497])
+    cur_ref, b_ref, it_ref, info = circuit_bdf_cpu(
+        km, lib, [dict(region=3, turns=300, area=4.8e-4, dir=em.js_dir)],
+        [dict(R=5.0, L=0.0, V=V)], steps, probe, order=order, tol=0.0, rel_tol=1e-10)
+    asm, space, fixed = _engine(km.P, km.T, km.RegionTags)
+    solver = (TransientBDF2Solver if order == 2 else TransientBDF1Solver)(asm)
+    solver.Tolerance = 0.0
+    solver.RelTolerance = 1e-10
+    w = Winding("Primary", 3, 300, 5.0, 4.8e-4)
+    w.setDirectionField(em.js_dir.T)
+    x, res = solver.solve(space, None, lib, {1: 0.0, 2: 0.0, 3: 0.0}, Circuit(5.0, 0.0, V), w,
+                          steps, fixed, None, None, None, probe)
+    cur = np.atleast_1d(res["CurrentHistory"])
+    print(f"BDF{order}: I {np.round(cur, 5)}\n  ref {np.round(cur_ref.ravel(), 5)}\n"
+          f"  |B| {np.round(res['ProbeB_History'], 5)}\n  ref {np.round(b_ref, 5)}\n"
+          f"  Newton {solver.NewtonIterations} (ref {it_ref}), PCG {solver.PcgIterations}",
+          flush=True)
+    assert abs(solver._scale[0] - info["scale"][0]) <= 1e-12 * info["scale"][0]
+    assert np.abs(cur - cur_ref.ravel()).max() <= 1e-6 * np.abs(cur_ref).max()
+    assert np.abs(res["ProbeB_History"] - b_ref).max() <= 1e-6 * np.abs(b_ref).max()
+
+
+def _ecore_setup():
+    from magpackage_amd.circuit import Circuit, Winding
+    from magpackage_amd.coil import CoilGeometryUtils as CG
+    from magpackage_amd.material import create_linear, create_nonlinear
+    d = np.load(GOLDEN / "ecore_mesh.npz")
+    tags = d["tags"].copy()
+    tags[np.isin(tags, [5, 6, 8, 9])] = 50        # primary coil (run_ecore_sdc_compare.m:29-37)
+    tags[np.isin(tags, [3, 4, 7, 10])] = 60       # secondary coil
+    coil = SimpleNamespace(P=d["P"], T=d["T"], RegionTags=tags, NumElements=len(tags))
+    center, _, area, ax = CG.autoDetectCircular(coil, 50)
+    dirm = -CG.computeCircularDirection(coil, 50, center, ax)
+    lib = {1: create_linear(1.0), 2: create_nonlinear(B_DATA, H_DATA), 50: create_linear(1.0),
+           60: create_linear(1.0)}
+    w = Winding("Primary", 50, 3000, 8.0, area)
+    w.setDirectionField(dirm)
+    circ = Circuit(8.0, 0.0, lambda t: 1500.0 * np.sin(2 * np.pi * 50 * t))
+    return d, tags, lib, w, circ
+
+
+def test_ecore_inrush_against_recorded_run(gpu):
+    """The E-core inrush of tutorials/run_ecore_sdc_compare.m (1500 V, 8 ohm,
+    3000 turns, BDF2, dt = 1 ms, tolerances 1e-4) over the recorded 0.1 s,
+    deep into saturation (|B| at the origin up to 4.3 T, current up to 38 A):
+    current and probe |B| histories against the reference's recording."""
+    from magpackage_amd.circuit import TransientBDF2Solver
+    d, tags, lib, w, circ = _ecore_setup()
+    ref = np.load(GOLDEN / "ecore_circuit_runs.npz")
+    t_ref, i_ref, b_ref = ref["dt1e-3_times"], ref["dt1e-3_current"], ref["dt1e-3_probeB"]
+    asm, space, fixed = _engine(d["P"], d["T"], tags)
+    solver = TransientBDF2Solver(asm)
+    solver.Tolerance = 1e-4
+    solver.RelTolerance = 1e-4
+    sig = {t: 0.0 for t in (1, 2, 50, 60)}
+
+    def monitor(t, I, tv, Iv, b, bv):
+        k = len(tv) - 1
+        if k % 10 == 0:
+            print(f"  step {k + 1}: t {t:.3f} I {I[0]:9.4f} (ref {i_ref[k]:9.4f}) |B| {b:.4f} "
+                  f"(ref {b_ref[k]:.4f}) Newton {solver.NewtonIterations[-1]} "
+                  f"PCG {solver.PcgIterations[-1]}", flush=True)
+
+    x, res = solver.solve(space, None, lib, sig, circ, w, [1e-3] * t_ref.size, fixed, None,
+                          None, monitor, np.zeros(3))
+    cur = np.atleast_1d(res["CurrentHistory"])
+    bh = res["ProbeB_History"]
+    ei = np.abs(cur - i_ref).max() / np.abs(i_ref).max()
+    eb = np.abs(bh - b_ref).max() / np.abs(b_ref).max()
+    print(f"max |dI| / max|I| = {ei:.2e}, max |dB| / max|B| = {eb:.2e}; first steps "
+          f"I {cur[:3]} vs {i_ref[:3]}; Newton total {sum(solver.NewtonIterations)}, "
+          f"PCG total {sum(solver.PcgIterations)}", flush=True)
+    # measured: 4.2e-9 (current) and 1.9e-9 (|B|) over the 100 steps
+    assert np.abs(cur[:2] - i_ref[:2]).max() <= 1e-8 * np.abs(i_ref[:2]).max()
+    assert ei <= 1e-7 and eb <= 1e-7
