@@ -1,8 +1,19 @@
 """Writes tests/golden/ecore_circuit_runs.npz: the reference's recorded
-field-circuit transients of the E-core (data/V-1500-R-8-N-3000-step-1e3.mat
-and data/V-1500-R-8-N-3000-step-5e4.mat: tutorials/run_ecore_sdc_compare.m's
-TransientBDF2Solver run, 1500 V / 50 Hz source, R = 8 ohm, 3000 turns,
-dt = 1 ms and 0.5 ms over 0.1 s, probe |B| at the origin).
+field-circuit transients of the E-core (TransientBDF2Solver on
+data/Ecore.mphtxt, primary coil of 3000 turns on a 1500 V / 50 Hz source,
+probe |B| at the origin):
+- dt1e-3, dt5e-4: data/V-1500-R-8-N-3000-step-{1e3,5e4}.mat,
+  tutorials/run_ecore_sdc_compare.m's run (R = 8 ohm, 24-point DW465-50
+  curve, tolerances 1e-4; dt = 1 ms and 0.5 ms over 0.1 s);
+- r10: data/R-10-step-1e3-tol-1e3-2.mat (R = 10 ohm, dt = 1 ms, 40 steps,
+  tolerances 1e-3, the 23-point curve of tutorials/run_ecore_transient.m);
+- decay_r5: data/shuaijian-R-5-step-1e3-tol-1e3-5.mat (R = 5 ohm, dt = 1 ms,
+  100 steps, tolerances 1e-3, 23-point curve, source switched off at
+  t = 10 ms as run_ecore_sdc_compare.m's timeDisc does).
+The file names give R, dt and the tolerance; the curve, the voltage and the
+switch-off time of the last two were identified by matching the recordings
+(each candidate among the tutorials' choices either reproduces a run to
+~1e-10 or departs from it within a few steps).
 
 Run in a checkout that has the reference at /root/reference:
 
@@ -20,7 +31,8 @@ import scipy.io as sio
 
 REF = Path("/root/reference/data")
 OUT = Path(__file__).resolve().parent / "ecore_circuit_runs.npz"
-RUNS = {"dt1e-3": "V-1500-R-8-N-3000-step-1e3.mat", "dt5e-4": "V-1500-R-8-N-3000-step-5e4.mat"}
+RUNS = {"dt1e-3": "V-1500-R-8-N-3000-step-1e3.mat", "dt5e-4": "V-1500-R-8-N-3000-step-5e4.mat",
+        "r10": "R-10-step-1e3-tol-1e3-2.mat", "decay_r5": "shuaijian-R-5-step-1e3-tol-1e3-5.mat"}
 
 
 def main():

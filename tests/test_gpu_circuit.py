@@ -75,7 +75,7 @@ def test_circuit_solver_matches_saddle_point_restatement(gpu, order):
 
 
 def _ecore_setup():
-    from magpackage_amd.circuit import Circuit, Winding
+    from magpackage_amd.circuit import Winding
     from magpackage_amd.coil import CoilGeometryUtils as CG
     from magpackage_amd.material import create_linear, create_nonlinear
     d = np.load(GOLDEN / "ecore_mesh.npz")
@@ -89,24 +89,36 @@ def _ecore_setup():
            60: create_linear(1.0)}
     w = Winding("Primary", 50, 3000, 8.0, area)
     w.setDirectionField(dirm)
-    circ = Circuit(8.0, 0.0, lambda t: 1500.0 * np.sin(2 * np.pi * 50 * t))
-    return d, tags, lib, w, circ
+    return d, tags, lib, w
 
 
-@pytest.mark.parametrize("run,dt", [("dt1e-3", 1e-3), ("dt5e-4", 5e-4)])
-def test_ecore_inrush_against_recorded_run(gpu, run, dt):
-    """The E-core inrush of tutorials/run_ecore_sdc_compare.m (1500 V, 8 ohm,
-    3000 turns, BDF2, tolerances 1e-4; dt = 1 ms and 0.5 ms) over the recorded
-    0.1 s, deep into saturation (|B| at the origin up to 4.3 T, current up to
-    38 A): current and probe |B| histories against the reference's recordings."""
-    from magpackage_amd.circuit import TransientBDF2Solver
-    d, tags, lib, w, circ = _ecore_setup()
+# recorded run: (dt, R, tolerance, B-H points, source switch-off time)
+RUNS = {"dt1e-3": (1e-3, 8.0, 1e-4, 24, None), "dt5e-4": (5e-4, 8.0, 1e-4, 24, None),
+        "r10": (1e-3, 10.0, 1e-3, 23, None), "decay_r5": (1e-3, 5.0, 1e-3, 23, 0.01)}
+
+
+@pytest.mark.parametrize("run", list(RUNS))
+def test_ecore_inrush_against_recorded_run(gpu, run):
+    """The E-core inrush of tutorials/run_ecore_sdc_compare.m (1500 V, 3000
+    turns, BDF2) in the four recorded variants: R = 8 ohm at dt 1 ms and
+    0.5 ms (tolerances 1e-4, 0.1 s), R = 10 ohm (tolerances 1e-3, 40 ms), and
+    R = 5 ohm with the source switched off at 10 ms (the current's decay
+    through the saturated core, 0.1 s); |B| at the origin reaches 4.3 T and the
+    current 47 A.  Current and probe |B| histories against the recordings."""
+    from magpackage_amd.circuit import Circuit, TransientBDF2Solver
+    from magpackage_amd.material import create_nonlinear
+    dt, R, tol, npts, t_off = RUNS[run]
+    d, tags, lib, w = _ecore_setup()
+    lib[2] = create_nonlinear(B_DATA[:npts], H_DATA[:npts])
+    w.Resistance = R
+    circ = Circuit(R, 0.0, lambda t: 1500.0 * np.sin(2 * np.pi * 50 * t)
+                   * (1.0 if t_off is None or t < t_off else 0.0))
     ref = np.load(GOLDEN / "ecore_circuit_runs.npz")
     t_ref, i_ref, b_ref = ref[f"{run}_times"], ref[f"{run}_current"], ref[f"{run}_probeB"]
     asm, space, fixed = _engine(d["P"], d["T"], tags)
     solver = TransientBDF2Solver(asm)
-    solver.Tolerance = 1e-4
-    solver.RelTolerance = 1e-4
+    solver.Tolerance = tol
+    solver.RelTolerance = tol
     sig = {t: 0.0 for t in (1, 2, 50, 60)}
 
     def monitor(t, I, tv, Iv, b, bv):
@@ -125,6 +137,9 @@ def test_ecore_inrush_against_recorded_run(gpu, run, dt):
     print(f"max |dI| / max|I| = {ei:.2e}, max |dB| / max|B| = {eb:.2e}; first steps "
           f"I {cur[:3]} vs {i_ref[:3]}; Newton total {sum(solver.NewtonIterations)}, "
           f"PCG total {sum(solver.PcgIterations)}", flush=True)
-    # measured: 4.2e-9 (current) and 1.9e-9 (|B|) over the 100 steps
+    # measured: 4e-9 on the 1e-4-tolerance runs; 2e-6 on the 1e-3-tolerance
+    # ones, whose Newton stops are looser (a residual near the threshold can
+    # stop one iteration apart)
+    bound = 1e-7 if tol <= 1e-4 else 1e-5
     assert np.abs(cur[:2] - i_ref[:2]).max() <= 1e-8 * np.abs(i_ref[:2]).max()
-    assert ei <= 1e-7 and eb <= 1e-7
+    assert ei <= bound and eb <= bound
