@@ -904,7 +904,9 @@ int mpk_cocg_aux_solve(const int32_t* colptr, const int32_t* rowind, const doubl
                        fixed, n, b2, dinv, r, x2, pc);
     precondition();
     hipLaunchKernelGGL(k_c_beta, dim3(1), dim3(kB), 0, st, pa, pb, pc, g_vec, 1, scal);
-    hipLaunchKernelGGL(k_c_direction, dim3(g_vec), dim3(kB), 0, st, n, scal, z, p);
+    // p = z (not z + 0 * p: the workspace is uninitialised and may hold NaN)
+    hipLaunchKernelGGL(k_copy, dim3(grid_parts(2 * n, kB)), dim3(kB), 0, st, 2 * n,
+                       reinterpret_cast<const double*>(z), reinterpret_cast<double*>(p));
     MPK_HIP_TRY(hipGetLastError());
     double h[8];
     MPK_HIP_TRY(hipMemcpyAsync(h, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -36,7 +36,7 @@ def csystem(gpu):
     return torch, asm, space, K, M, Ms, fixed
 
 
-def _cocg(torch, asm, space, R, I, fixed, b, aux_c=None, tol=1e-12):
+def _cocg(torch, asm, space, R, I, fixed, b, aux_c=None, tol=1e-12, fill=None):
     from magpackage_amd import _lib as L
     from magpackage_amd.transient import build_aux_space
     plan = asm.plan(space)
@@ -52,6 +52,8 @@ def _cocg(torch, asm, space, R, I, fixed, b, aux_c=None, tol=1e-12):
     ws = C.c_int64()
     L.check(L.lib.mpk_cocg_aux_work_size(n, nv, C.byref(ws)))
     work = torch.empty(ws.value, dtype=torch.float64, device=dev)
+    if fill is not None:
+        work.fill_(fill)
     bd = torch.from_numpy(b.astype(np.complex128)).to(dev)
     x = torch.empty(n, dtype=torch.complex128, device=dev)
     it, rr = C.c_int(), C.c_double()
@@ -96,6 +98,22 @@ def test_cocg_against_direct_solve(csystem, omega):
     assert it_aux * 2 < it_jac
     x2, it2, _ = _cocg(torch, asm, space, R, I, fixed, b, aux_c=eps * M.values + I)
     assert it2 == it_aux and np.array_equal(x2.view(np.float64), x_aux.view(np.float64))
+
+
+def test_cocg_workspace_contents_do_not_matter(csystem):
+    """Regression: the workspace is uninitialised memory; a NaN-filled one
+    gives the same iterations and the same bits as a zero-filled one."""
+    torch, asm, space, K, M, Ms, fixed = csystem
+    eps = 1e-6 * float(np.abs(K.to_scipy().diagonal()).max())
+    R = K.values + eps * M.values
+    I = 314.0 * Ms.values
+    n = asm.plan(space).n
+    b = np.random.default_rng(8).normal(size=n) + 0.5j
+    x0, it0, r0 = _cocg(torch, asm, space, R, I, fixed, b, aux_c=eps * M.values + I, fill=0.0)
+    x1, it1, r1 = _cocg(torch, asm, space, R, I, fixed, b, aux_c=eps * M.values + I,
+                        fill=float("nan"))
+    assert it0 > 10 and r0 <= 1e-12
+    assert it1 == it0 and np.array_equal(x1.view(np.float64), x0.view(np.float64))
 
 
 def test_cocg_real_system_matches_pcg(csystem):
