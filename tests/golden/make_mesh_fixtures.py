@@ -1,5 +1,6 @@
-"""Writes tests/golden/{ecore,team7}_mesh.npz: the reference's COMSOL meshes
-(data/Ecore.mphtxt, data/Team7.mphtxt) read with magpackage_amd.mesh_io.read_mphtxt.
+"""Writes tests/golden/{ecore,team7,d800kva}_mesh.npz: the reference's COMSOL
+meshes (data/Ecore.mphtxt, data/Team7.mphtxt, data/D800kVA.mphtxt) read with
+magpackage_amd.mesh_io.read_mphtxt.
 
 Run in a checkout that has the reference at /root/reference:
 
@@ -46,7 +47,7 @@ def independent(path):
 
 
 def main():
-    for name in ("Ecore", "Team7"):
+    for name in ("Ecore", "Team7", "D800kVA"):
         path = REF / f"{name}.mphtxt"
         m = read_mphtxt(str(path))
         ind = independent(path)

@@ -74,6 +74,50 @@ def test_circuit_solver_matches_saddle_point_restatement(gpu, order):
     assert np.abs(res["ProbeB_History"] - b_ref).max() <= 1e-6 * np.abs(b_ref).max()
 
 
+def test_two_windings_match_saddle_point_restatement(gpu):
+    """Two windings (the small E-core box's coil split into halves at the
+    leg's mid-plane, each on its own source and resistor): the k x k Schur
+    step of the engine against the monolithic CPU solve, BDF2."""
+    from magpackage_amd.circuit import Circuit, TransientBDF2Solver, Winding
+    from magpackage_amd.material import create_linear, create_nonlinear
+    from magpackage_amd.meshgen import centroid_axis, ecore_box
+    em = ecore_box(8, 8, 6)
+    km = em.mesh
+    half = (km.RegionTags == 3) & (centroid_axis(km, 1) > 0.08)
+    km.RegionTags[half] = 4
+    d3 = np.where((km.RegionTags == 3)[:, None], em.js_dir, 0.0)
+    d4 = np.where((km.RegionTags == 4)[:, None], em.js_dir, 0.0)
+    lib = {1: create_linear(1.0), 2: create_nonlinear(B_DATA, H_DATA), 3: create_linear(1.0),
+           4: create_linear(1.0)}
+    V1 = lambda t: 90.0 * np.sin(2 * np.pi * 50 * t)           # noqa: E731
+    V2 = lambda t: 60.0 * np.sin(2 * np.pi * 50 * t + 0.7)     # noqa: E731
+    steps = [1e-3] * 5
+    probe = np.array([0.0503, 0.0811, 0.0497])
+    cur_ref, b_ref, it_ref, info = circuit_bdf_cpu(
+        km, lib, [dict(region=3, turns=200, area=4.8e-4, dir=d3),
+                  dict(region=4, turns=150, area=4.8e-4, dir=d4)],
+        [dict(R=5.0, L=0.0, V=V1), dict(R=3.0, L=0.02, V=V2)], steps, probe, order=2,
+        tol=0.0, rel_tol=1e-10)
+    asm, space, fixed = _engine(km.P, km.T, km.RegionTags)
+    solver = TransientBDF2Solver(asm)
+    solver.Tolerance = 0.0
+    solver.RelTolerance = 1e-10
+    w1 = Winding("A", 3, 200, 5.0, 4.8e-4)
+    w1.setDirectionField(d3.T)
+    w2 = Winding("B", 4, 150, 3.0, 4.8e-4)
+    w2.setDirectionField(d4.T)
+    x, res = solver.solve(space, None, lib, {t: 0.0 for t in (1, 2, 3, 4)},
+                          [Circuit(5.0, 0.0, V1), Circuit(3.0, 0.02, V2)], [w1, w2], steps,
+                          fixed, None, None, None, probe)
+    cur = res["CurrentHistory"]
+    print(f"two windings: I {np.round(cur, 5).tolist()}\n  ref {np.round(cur_ref, 5).tolist()}"
+          f"\n  Newton {solver.NewtonIterations} (ref {it_ref})", flush=True)
+    assert cur.shape == cur_ref.shape == (5, 2)
+    assert np.allclose(solver._scale, info["scale"], rtol=1e-12, atol=0)
+    assert np.abs(cur - cur_ref).max() <= 1e-6 * np.abs(cur_ref).max()
+    assert np.abs(res["ProbeB_History"] - b_ref).max() <= 1e-6 * np.abs(b_ref).max()
+
+
 def _ecore_setup():
     from magpackage_amd.circuit import Winding
     from magpackage_amd.coil import CoilGeometryUtils as CG
@@ -143,3 +187,49 @@ def test_ecore_inrush_against_recorded_run(gpu, run):
     bound = 1e-7 if tol <= 1e-4 else 1e-5
     assert np.abs(cur[:2] - i_ref[:2]).max() <= 1e-8 * np.abs(i_ref[:2]).max()
     assert ei <= bound and eb <= bound
+
+
+def test_d800kva_two_primaries(gpu):
+    """tutorials/run_D800KVA_transient.m on the reference's transformer mesh
+    (65,394 tets): two primaries of 2933 turns on mirror-image legs, 49.5 kV /
+    50 Hz through 23.42 ohm each, BDF2 dt = 0.5 ms, one cycle into
+    saturation.  No recording of this run ships with the reference, so the
+    check is physical: every step converges, and the mirror-symmetric
+    primaries carry equal currents to within 1% of the peak (the mesh is not
+    exactly symmetric) while the peak reaches the saturated inrush range."""
+    from magpackage_amd.circuit import Circuit, TransientBDF2Solver, Winding
+    from magpackage_amd.coil import CoilGeometryUtils as CG
+    from magpackage_amd.material import create_linear, create_nonlinear
+    d = np.load(GOLDEN / "d800kva_mesh.npz")
+    tags = d["tags"].copy()
+    for src, dst in (([5, 6, 10, 13], 50), ([14, 15, 19, 22], 51), ([7, 8, 11, 12], 60),
+                     ([16, 17, 20, 21], 61)):
+        tags[np.isin(tags, src)] = dst        # run_D800KVA_transient.m:29-43
+    m = SimpleNamespace(P=d["P"], T=d["T"], RegionTags=tags, NumElements=len(tags))
+    # run_D800KVA_transient.m:49-55
+    bd = B_DATA[:22] + [2.118163704, 2.23]
+    hd = H_DATA[:16] + [100.17, 137.5, 295.39, 1014.01, 2500, 12500, 112500, 212500]
+    nl = create_nonlinear(bd, hd)
+    lib = {t: nl for t in (2, 3, 4, 9, 18, 23)}
+    for t in (1, 50, 51, 60, 61):
+        lib[t] = create_linear(1.0)
+    c1, _, s1, a1 = CG.autoDetectCircular(m, 50)
+    c2, _, s2, a2 = CG.autoDetectCircular(m, 51)
+    w1 = Winding("Primary_1", 50, 2933, 23.42, s1)
+    w1.setDirectionField(CG.computeCircularDirection(m, 50, c1, a1))
+    w2 = Winding("Primary_2", 51, 2933, 23.42, s2)
+    w2.setDirectionField(-CG.computeCircularDirection(m, 51, c2, a2))
+    V = lambda t: 49497.0 * np.sin(2 * np.pi * 50 * t)    # noqa: E731
+    asm, space, fixed = _engine(d["P"], d["T"], tags)
+    solver = TransientBDF2Solver(asm)
+    x, res = solver.solve(space, None, lib, {t: 0.0 for t in lib},
+                          [Circuit(23.42, 0.0, V), Circuit(23.42, 0.0, V)], [w1, w2], [5e-4] * 40,
+                          fixed, None, None, None, np.array([-0.25, 0.0, 0.0]))
+    cur = res["CurrentHistory"]
+    peak = np.abs(cur).max()
+    print(f"D800kVA: peak currents {np.abs(cur).max(axis=0)}, max |I1 - I2| "
+          f"{np.abs(cur[:, 0] - cur[:, 1]).max():.4f} A, Newton {sum(solver.NewtonIterations)}, "
+          f"PCG {sum(solver.PcgIterations)}", flush=True)
+    assert max(solver.NewtonIterations) < solver.MaxIterations
+    assert np.abs(cur[:, 0] - cur[:, 1]).max() <= 0.01 * peak
+    assert 20.0 < peak < 200.0

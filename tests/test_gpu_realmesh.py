@@ -1,5 +1,6 @@
 """GPU parity on the reference's own COMSOL meshes (tests/golden/ecore_mesh.npz,
-team7_mesh.npz, read from data/Ecore.mphtxt and data/Team7.mphtxt): unstructured
+team7_mesh.npz, d800kva_mesh.npz, read from data/Ecore.mphtxt, data/Team7.mphtxt
+and data/D800kVA.mphtxt): unstructured
 tetrahedra with up to 9 elements around an edge, where the Kuhn boxes of the
 other tests have at most 6 (one gather round of the ring driver).
 
@@ -38,7 +39,7 @@ class _Raw:
         self.num_elems = self.T.shape[0]
 
 
-@pytest.fixture(scope="module", params=["ecore", "team7"])
+@pytest.fixture(scope="module", params=["ecore", "team7", "d800kva"])
 def real(request, gpu):
     from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
     raw = _Raw(np.load(GOLDEN / f"{request.param}_mesh.npz"))
@@ -83,7 +84,7 @@ def test_real_edges_dofs_pattern(real):
     assert np.array_equal(colptr.cpu().numpy(), c)
     assert np.array_equal(rowind.cpu().numpy(), r)
     # the ring records need every edge's elements to form one ring or chain
-    # (a manifold neighbourhood); both meshes are
+    # (a manifold neighbourhood); all three meshes are
     assert plan.ring_order
     # edges with more incident elements than one 6-incidence gather round holds
     assert np.bincount(mi["t2e"].ravel()).max() > 6

@@ -107,7 +107,8 @@ def test_read_errors(tmp_path):
 
 
 @pytest.mark.parametrize("name,nn,ne,nf", [("ecore", 4324, 23569, 0),
-                                            ("team7", 10007, 57892, 5936)])
+                                            ("team7", 10007, 57892, 5936),
+                                            ("d800kva", 11225, 65394, 0)])
 def test_reference_mesh_fixtures(name, nn, ne, nf):
     """The counts the files' headers state; every node used; a closed
     tetrahedral volume (Euler characteristic of a ball: V - E + F - T = 1)."""
