@@ -150,6 +150,31 @@ __device__ __forceinline__ TileRange tile_begin(const TileCtx& tc, Op& op, Acc* 
     return tile_begin_at(tc, op, acc, smem, xcd_tile(blockIdx.x, gridDim.x));
 }
 
+// Output values are written once and never read back by the kernel: the
+// stores carry the non-temporal hint, so the output stream passes the caches
+// that hold the gathered coordinates and element data.
+#ifndef MPK_NT_STORES
+#define MPK_NT_STORES 1
+#endif
+__device__ __forceinline__ void st_out(double* p, double v) {
+#if MPK_NT_STORES
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+__device__ __forceinline__ void st_out(double2* p, double2 v) {
+#if MPK_NT_STORES
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    d2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<d2v*>(p));
+#else
+    *p = v;
+#endif
+}
+
 // Ops may store a finished tile themselves (store_tile(base, acc, size)),
 // e.g. to order the stores of several output arrays.
 template <class Op, class = void>

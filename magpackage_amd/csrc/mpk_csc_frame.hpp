@@ -154,7 +154,7 @@ struct FrameCurl {
     __device__ void block_init(unsigned char*) const {}
     __device__ Col col_begin() const { return Col{}; }
     __device__ void col_end(int64_t, Col&) const {}
-    __device__ void store(int64_t p, Acc a) const { out[p] = a; }
+    __device__ void store(int64_t p, Acc a) const { st_out(out + p, a); }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return nu[e]; }
     __device__ void compute(const FrameNodes& nodes, const Elem& nuv, uint32_t f, const FPos& pos,
@@ -184,7 +184,7 @@ struct FrameMass {
     __device__ void block_init(unsigned char*) const {}
     __device__ Col col_begin() const { return Col{}; }
     __device__ void col_end(int64_t, Col&) const {}
-    __device__ void store(int64_t p, Acc a) const { out[p] = a; }
+    __device__ void store(int64_t p, Acc a) const { st_out(out + p, a); }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return coeffs ? coeffs[e] : 1.0; }
     __device__ void compute(const FrameNodes& nodes, const Elem& sigma, uint32_t f,
@@ -221,7 +221,7 @@ struct FrameKM {
     __device__ void col_end(int64_t, Col&) const {}
     __device__ void store(int64_t p, Acc a) const {
         // K + 1j*omega*M: real part K, imaginary part omega * (sum of M)
-        out[p] = make_double2(a.x, omega * a.y);
+        st_out(out + p, make_double2(a.x, omega * a.y));
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return make_double2(nu[e], sigma[e]); }
@@ -308,7 +308,7 @@ struct FrameJac {
     __device__ Col col_begin() const { return Col{0.0}; }
     __device__ void col_end(int64_t j, Col& cs) const { res[j] = cs.r; }
     __device__ void store(int64_t p, Acc a) const {
-        if (kCalcJ) out[p] = a;
+        if (kCalcJ) st_out(out + p, a);
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
@@ -371,18 +371,19 @@ struct FrameNewton {
     }
     __device__ void store_planes(int64_t p, const Acc* a, int size) const {
         const double k = a[0];
-        if (out_k) out_k[p] = k;
-        if (kWithM) out_m[p] = a[size];
-        if (out_j) out_j[p] = k + a[kD * size];
+        if (out_k) st_out(out_k + p, k);
+        if (kWithM) st_out(out_m + p, a[size]);
+        if (out_j) st_out(out_j + p, k + a[kD * size]);
     }
     // array-outer: each output's tile range is written as one contiguous run
     __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
         if (out_k)
-            for (int i = threadIdx.x; i < size; i += kW) out_k[base + i] = acc[i];
+            for (int i = threadIdx.x; i < size; i += kW) st_out(out_k + base + i, acc[i]);
         if (kWithM)
-            for (int i = threadIdx.x; i < size; i += kW) out_m[base + i] = acc[size + i];
+            for (int i = threadIdx.x; i < size; i += kW) st_out(out_m + base + i, acc[size + i]);
         if (out_j)
-            for (int i = threadIdx.x; i < size; i += kW) out_j[base + i] = acc[i] + acc[kD * size + i];
+            for (int i = threadIdx.x; i < size; i += kW)
+                st_out(out_j + base + i, acc[i] + acc[kD * size + i]);
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
@@ -456,18 +457,18 @@ struct FrameNewtonLin {
         if (res) res[j] = cs.r;
     }
     __device__ void store(int64_t p, Acc k) const {
-        if (out_k) out_k[p] = k;
-        if (out_m) out_m[p] = 0.0;
-        if (out_j) out_j[p] = k + 0.0;
+        if (out_k) st_out(out_k + p, k);
+        if (out_m) st_out(out_m + p, 0.0);
+        if (out_j) st_out(out_j + p, k + 0.0);
     }
     // array-outer: each output's tile range is written as one contiguous run
     __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
         if (out_k)
-            for (int i = threadIdx.x; i < size; i += kW) out_k[base + i] = acc[i];
+            for (int i = threadIdx.x; i < size; i += kW) st_out(out_k + base + i, acc[i]);
         if (out_m)
-            for (int i = threadIdx.x; i < size; i += kW) out_m[base + i] = 0.0;
+            for (int i = threadIdx.x; i < size; i += kW) st_out(out_m + base + i, 0.0);
         if (out_j)
-            for (int i = threadIdx.x; i < size; i += kW) out_j[base + i] = acc[i] + 0.0;
+            for (int i = threadIdx.x; i < size; i += kW) st_out(out_j + base + i, acc[i] + 0.0);
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load_bnu(e); }
@@ -733,7 +734,8 @@ struct FrameKMF {
     __device__ Col col_begin() const { return Col{}; }
     __device__ void col_end(int64_t, Col&) const {}
     __device__ void store(int64_t p, Acc a) const {
-        for (int f = 0; f < nfreq; ++f) km.out[f * nnz + p] = make_double2(a.x, omegas[f] * a.y);
+        for (int f = 0; f < nfreq; ++f)
+            st_out(km.out + f * nnz + p, make_double2(a.x, omegas[f] * a.y));
     }
     // frequency-outer: each value set's tile range is written as one
     // contiguous run before the next set's
@@ -743,7 +745,7 @@ struct FrameKMF {
             double2* __restrict__ o = km.out + f * nnz + base;
             for (int i = threadIdx.x; i < size; i += kW) {
                 const Acc a = acc[i];
-                o[i] = make_double2(a.x, w * a.y);
+                st_out(o + i, make_double2(a.x, w * a.y));
             }
         }
     }
