@@ -1,0 +1,95 @@
+"""The CPU restatement of the reference's field-circuit solvers
+(tests/circuit_ref.py) pinned against a closed form.
+
+With linear materials and sigma = 0 the gauged field is linear in the winding
+current: W^T A = L I, with L = W~^T K^-1 W~. Here W~ is the winding vector
+projected onto divergence-free loads, and K the Dirichlet-reduced curl-curl
+operator on gauged fields. Every BDF step then reduces to a scalar RL
+recurrence:
+  BDF1: R I_n + L (I_n - I_{n-1}) / dt = V(t_n)
+  BDF2: R I_n + L (1.5 I_n - 2 I_{n-1} + 0.5 I_{n-2}) / dt = V(t_n),
+with BDF1 on the first step.
+
+L is computed here by a gauge-free route: a penalised curl-curl solve,
+independent of the restatement's saddle point."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import oracle as O
+from magpackage_amd.material import create_linear
+from magpackage_amd.meshgen import ecore_box
+from tests.circuit_ref import boundary_nodes, circuit_bdf_cpu
+from tests.helpers import mass_tables, stiffness_tables
+
+
+def _inductance(km, nu, region, turns, area, direction):
+    """L = W^T A for K A = W with the gauge G^T M A = 0, solved as
+    (K + gamma M G Ln^-1 G^T M) A = W~ on the free edges (SPD; equals K on
+    gauged fields)."""
+    mi = O.mesh_inputs(km)
+    n = mi["ndof"]
+    P, T, D, S = mi["P"], mi["T"], mi["Dofs"], mi["Signs"]
+    qw, curl, _ = stiffness_tables()
+    qm, val, _ = mass_tables(2)
+    K = O.sparse_scipy(*O.curl_curl(P, T, D, S, nu, qw, curl), n, n, drop_zeros=False)
+    M = O.sparse_scipy(*O.mass(P, T, D, S, None, qm, val), n, n, drop_zeros=False)
+    Iw, Vw = O.winding(P, T, D, S, km.RegionTags.astype(np.float64), region, direction,
+                       np.zeros(3), turns / area, qm, val)
+    W = np.zeros(n)
+    ok = Iw > 0
+    np.add.at(W, Iw[ok].astype(np.int64) - 1, Vw[ok])
+    edges = mi["edges"]
+    emap = np.zeros(edges.shape[0], dtype=np.int64)
+    emap[mi["t2e"].ravel() - 1] = mi["cell"].ravel()
+    fixed = np.zeros(n, dtype=bool)
+    fd = emap[O.boundary_edges(km.T, edges)]
+    fixed[fd[fd > 0] - 1] = True
+    free_nodes = np.setdiff1d(np.arange(km.P.shape[0]), boundary_nodes(km.T))
+    fid = np.full(km.P.shape[0], -1)
+    fid[free_nodes] = np.arange(free_nodes.size)
+    rows = emap - 1
+    a, b = fid[edges[:, 0] - 1], fid[edges[:, 1] - 1]
+    ma, mb = (a >= 0) & (rows >= 0), (b >= 0) & (rows >= 0)
+    G = sp.csr_matrix((np.r_[-np.ones(ma.sum()), np.ones(mb.sum())],
+                       (np.r_[rows[ma], rows[mb]], np.r_[a[ma], b[mb]])),
+                      shape=(n, free_nodes.size))
+    f = ~fixed
+    Kf, Mf, Gf, Wf = K[f][:, f], M[f][:, f], G[f], W[f]
+    Ln = (Gf.T @ Mf @ Gf).toarray()
+    MG = (Mf @ Gf).toarray()
+    Wt = Wf - MG @ np.linalg.solve(Ln, Gf.T @ Wf)
+    # the penalty vanishes on gauged fields for any weight; this one matches K's scale
+    gamma = Kf.diagonal().mean() / Mf.diagonal().mean()
+    A = np.linalg.solve(Kf.toarray() + gamma * MG @ np.linalg.solve(Ln, MG.T), Wt)
+    # gauged: G^T M A = 0 up to rounding of its own terms
+    assert np.all(np.abs(MG.T @ A) <= 1e-9 * (np.abs(MG.T) @ np.abs(A)))
+    return float(Wf @ A)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_linear_circuit_reduces_to_rl_recurrence(order):
+    em = ecore_box(6, 6, 4)
+    km = em.mesh
+    mu0 = 4e-7 * np.pi
+    lib = {1: create_linear(1.0), 2: create_linear(500.0), 3: create_linear(1.0)}
+    nu = np.where(km.RegionTags == 2, 1.0 / (500.0 * mu0), 1.0 / mu0)
+    turns, area, R = 200.0, 4.8e-4, 3.0
+    V = lambda t: 50.0 * np.sin(2 * np.pi * 50 * t)      # noqa: E731
+    dt, nsteps = 1e-3, 6
+    cur, _, its, _ = circuit_bdf_cpu(km, lib, [dict(region=3, turns=turns, area=area,
+                                                    dir=em.js_dir)],
+                                     [dict(R=R, L=0.0, V=V)], [dt] * nsteps, np.zeros(3),
+                                     order=order, tol=0.0, rel_tol=1e-12)
+    Lind = _inductance(km, nu, 3, turns, area, em.js_dir)
+    I = np.zeros(nsteps + 2)
+    for k in range(nsteps):
+        t = (k + 1) * dt
+        if order == 1 or k == 0:
+            I[k + 2] = (V(t) + Lind * I[k + 1] / dt) / (R + Lind / dt)
+        else:
+            I[k + 2] = (V(t) + Lind * (2 * I[k + 1] - 0.5 * I[k]) / dt) / (R + 1.5 * Lind / dt)
+    ref = I[2:]
+    print(f"BDF{order}: L = {Lind:.6e} H, restatement {cur.ravel()}, recurrence {ref}")
+    assert np.abs(cur.ravel() - ref).max() <= 1e-8 * np.abs(ref).max()
