@@ -8,6 +8,14 @@ field:
 - ``probeB(A_sol, point)`` (:403-441): B in the tet that holds the point, or 0
   outside the mesh.
 - ``probeLine(A_sol, start, end, numPts, "B")`` (:505-532).
+- ``probeSolution(A_sol, point)`` (:443-454, :612-635): A interpolated with the
+  edge basis at the point. The reference's element's basis values are mapped by
+  the covariant Piola map, x_ref = J^-1 (x - p1) and N = N_ref J^-1.
+- ``probeTotalCurrent(point, A_sol, V_sol, omega, sigmaMap, sourceMap)``
+  (:456-503): J = Js + sigma (-1j omega A - grad V) in conducting elements, Js
+  elsewhere. With the A* gauge that FrequencySolver returns, V_sol is zero and
+  J_eddy = -1j omega sigma A*. ``regions`` (an extension) restricts the point
+  location to the given tags, e.g. to probe a conductor on its surface.
 
 The reference locates points with MATLAB's ``triangulation.pointLocation``.
 Here every tet's affine map is inverted once on the host. A point's reference
@@ -63,17 +71,64 @@ class PostProcessor:
                 cols.append(self.Assembler.computeElementB(col.contiguous(), space).T)
         return torch.stack(cols, dim=2)
 
-    def locate(self, points) -> np.ndarray:
-        """0-based tet index holding each point (N, 3), -1 outside the mesh."""
+    def locate(self, points, regions=None) -> np.ndarray:
+        """0-based tet index holding each point (N, 3), -1 outside the mesh
+        (or outside the given region tags)."""
         pts = np.atleast_2d(np.asarray(points, dtype=np.float64))
         out = np.full(pts.shape[0], -1, dtype=np.int64)
+        allowed = (None if regions is None
+                   else np.isin(np.asarray(self.Mesh.RegionTags), np.atleast_1d(regions)))
         for i, p in enumerate(pts):
             x = np.einsum("eij,ej->ei", self._Jinv, p[None, :] - self._p0)
             lam_min = np.minimum(x.min(axis=1), 1.0 - x.sum(axis=1))
-            hit = np.nonzero(lam_min >= -1e-12)[0]
+            ok = lam_min >= -1e-12
+            if allowed is not None:
+                ok &= allowed
+            hit = np.nonzero(ok)[0]
             if hit.size:
                 out[i] = hit[0]
         return out
+
+    def _probe_edge_field(self, sol: np.ndarray, e: int, point) -> np.ndarray:
+        """internalProbeNedelec (:612-635): sum_k sol[dof_k] s_k N_k(point)."""
+        from .fem_tables import nedelec_tet_p1
+        xr = self._Jinv[e] @ (np.asarray(point, dtype=np.float64) - self._p0[e])
+        val, _ = nedelec_tet_p1(xr.reshape(3, 1))           # (6, 3, 1)
+        N = val[:, :, 0] @ self._Jinv[e]                    # rows: physical basis
+        key = FunctionSpace("Nedelec", 1).toString()
+        dofs = self.DofHandler.DofMaps[key][e].cpu().numpy().astype(np.int64)
+        signs = self.Mesh.T2E_Sign[e].cpu().numpy().astype(np.float64)
+        coef = np.where(dofs > 0, sol[np.maximum(dofs, 1) - 1], 0.0) * signs
+        return coef @ N
+
+    def probeSolution(self, A_sol, point):
+        """(A_val (3,), elem_idx 1-based; NaN and zeros outside the mesh)."""
+        e = int(self.locate(point)[0])
+        if e < 0:
+            return np.zeros(3), float("nan")
+        a = A_sol.detach().cpu().numpy() if torch.is_tensor(A_sol) else np.asarray(A_sol)
+        return self._probe_edge_field(a, e, point), e + 1
+
+    def probeTotalCurrent(self, point, A_sol, V_sol, omega, sigmaMap, sourceMap,
+                          regions=None):
+        """(J_total (3,), elem_idx 1-based).  V_sol must be None or zero: only
+        the edge field's gauge (A*) is solved here."""
+        from .assembler import expand_material_map, expand_source_map
+        e = int(self.locate(point, regions)[0])
+        if e < 0:
+            return np.zeros(3), float("nan")
+        if V_sol is not None:
+            v = V_sol.detach().cpu().numpy() if torch.is_tensor(V_sol) else np.asarray(V_sol)
+            if np.any(v != 0):
+                raise NotImplementedError("a nonzero scalar potential is not supported")
+        tags = np.asarray(self.Mesh.RegionTags)
+        js = expand_source_map(sourceMap, tags)[e]
+        sigma = expand_material_map(sigmaMap, tags)[e]
+        if abs(sigma) <= 1e-12:
+            return js.astype(np.complex128), e + 1
+        a = A_sol.detach().cpu().numpy() if torch.is_tensor(A_sol) else np.asarray(A_sol)
+        A = self._probe_edge_field(a, e, point)
+        return js + sigma * (-1j * omega * A), e + 1
 
     def probeB(self, A_sol, point):
         """(B_val, elem_idx): B (3,) or (3, Ncols) at the point, elem_idx
