@@ -379,7 +379,7 @@ class TransientSolver:
             if extrap and warm_start is None:
                 x, label = 2.0 * x_prev - x_prev2, "extrapolated"
             if warm_start is not None and self.reg_abs > 0:
-                # the residual judges: warm start, extrapolation or x_prev
+                # the residual decides between warm start, extrapolation or x_prev
                 _, r_prev = self.residual(x_prev, b, dt)
                 ctx["r_prev"] = r_prev
                 rel_ref = r_prev  # the plain method's first residual
