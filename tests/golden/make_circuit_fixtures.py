@@ -9,7 +9,10 @@ probe |B| at the origin):
   tolerances 1e-3, the 23-point curve of tutorials/run_ecore_transient.m);
 - decay_r5: data/shuaijian-R-5-step-1e3-tol-1e3-5.mat (R = 5 ohm, dt = 1 ms,
   100 steps, tolerances 1e-3, 23-point curve, source switched off at
-  t = 10 ms as run_ecore_sdc_compare.m's timeDisc does).
+  t = 10 ms as run_ecore_sdc_compare.m's timeDisc does);
+- decay_r02_1e2, decay_r02_1e3: data/shuaijian-R-0.2-step1-5e4-step2-1e-{2,3}-
+  tol-1e3-5.mat (R = 0.2 ohm, the same switch-off, 20 steps of 0.5 ms then
+  steps of 10 ms or 1 ms to 0.1 s: BDF2 across a step-size change).
 The file names give R, dt and the tolerance; the curve, the voltage and the
 switch-off time of the last two were identified by matching the recordings
 (each candidate among the tutorials' choices either reproduces a run to
@@ -32,7 +35,9 @@ import scipy.io as sio
 REF = Path("/root/reference/data")
 OUT = Path(__file__).resolve().parent / "ecore_circuit_runs.npz"
 RUNS = {"dt1e-3": "V-1500-R-8-N-3000-step-1e3.mat", "dt5e-4": "V-1500-R-8-N-3000-step-5e4.mat",
-        "r10": "R-10-step-1e3-tol-1e3-2.mat", "decay_r5": "shuaijian-R-5-step-1e3-tol-1e3-5.mat"}
+        "r10": "R-10-step-1e3-tol-1e3-2.mat", "decay_r5": "shuaijian-R-5-step-1e3-tol-1e3-5.mat",
+        "decay_r02_1e2": "shuaijian-R-0.2-step1-5e4-step2-1e-2-tol-1e3-5.mat",
+        "decay_r02_1e3": "shuaijian-R-0.2-step1-5e4-step2-1e-3-tol-1e3-5.mat"}
 
 
 def main():

@@ -136,22 +136,28 @@ def _ecore_setup():
     return d, tags, lib, w
 
 
-# recorded run: (dt, R, tolerance, B-H points, source switch-off time)
-RUNS = {"dt1e-3": (1e-3, 8.0, 1e-4, 24, None), "dt5e-4": (5e-4, 8.0, 1e-4, 24, None),
-        "r10": (1e-3, 10.0, 1e-3, 23, None), "decay_r5": (1e-3, 5.0, 1e-3, 23, 0.01)}
+# recorded run: (steps, R, tolerance, B-H points, source switch-off time)
+RUNS = {"dt1e-3": ([1e-3] * 100, 8.0, 1e-4, 24, None),
+        "dt5e-4": ([5e-4] * 200, 8.0, 1e-4, 24, None),
+        "r10": ([1e-3] * 40, 10.0, 1e-3, 23, None),
+        "decay_r5": ([1e-3] * 100, 5.0, 1e-3, 23, 0.01),
+        "decay_r02_1e2": ([5e-4] * 20 + [1e-2] * 9, 0.2, 1e-3, 23, 0.01),
+        "decay_r02_1e3": ([5e-4] * 20 + [1e-3] * 90, 0.2, 1e-3, 23, 0.01)}
 
 
 @pytest.mark.parametrize("run", list(RUNS))
 def test_ecore_inrush_against_recorded_run(gpu, run):
     """The E-core inrush of tutorials/run_ecore_sdc_compare.m (1500 V, 3000
-    turns, BDF2) in the four recorded variants: R = 8 ohm at dt 1 ms and
-    0.5 ms (tolerances 1e-4, 0.1 s), R = 10 ohm (tolerances 1e-3, 40 ms), and
+    turns, BDF2) in the six recorded variants: R = 8 ohm at dt 1 ms and
+    0.5 ms (tolerances 1e-4, 0.1 s), R = 10 ohm (tolerances 1e-3, 40 ms),
     R = 5 ohm with the source switched off at 10 ms (the current's decay
-    through the saturated core, 0.1 s); |B| at the origin reaches 4.3 T and the
-    current 47 A.  Current and probe |B| histories against the recordings."""
+    through the saturated core, 0.1 s), and R = 0.2 ohm with the switch-off
+    and a step change from 0.5 ms to 10 ms or 1 ms there; |B| at the origin
+    reaches 4.3 T and the current 47 A.  Current and probe |B| histories
+    against the recordings."""
     from magpackage_amd.circuit import Circuit, TransientBDF2Solver
     from magpackage_amd.material import create_nonlinear
-    dt, R, tol, npts, t_off = RUNS[run]
+    steps, R, tol, npts, t_off = RUNS[run]
     d, tags, lib, w = _ecore_setup()
     lib[2] = create_nonlinear(B_DATA[:npts], H_DATA[:npts])
     w.Resistance = R
@@ -172,7 +178,8 @@ def test_ecore_inrush_against_recorded_run(gpu, run):
                   f"(ref {b_ref[k]:.4f}) Newton {solver.NewtonIterations[-1]} "
                   f"PCG {solver.PcgIterations[-1]}", flush=True)
 
-    x, res = solver.solve(space, None, lib, sig, circ, w, [dt] * t_ref.size, fixed, None,
+    assert len(steps) == t_ref.size and np.allclose(np.cumsum(steps), t_ref, rtol=0, atol=1e-12)
+    x, res = solver.solve(space, None, lib, sig, circ, w, steps, fixed, None,
                           None, monitor, np.zeros(3))
     cur = np.atleast_1d(res["CurrentHistory"])
     bh = res["ProbeB_History"]
