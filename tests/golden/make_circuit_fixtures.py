@@ -12,7 +12,12 @@ probe |B| at the origin):
   t = 10 ms as run_ecore_sdc_compare.m's timeDisc does);
 - decay_r02_1e2, decay_r02_1e3: data/shuaijian-R-0.2-step1-5e4-step2-1e-{2,3}-
   tol-1e3-5.mat (R = 0.2 ohm, the same switch-off, 20 steps of 0.5 ms then
-  steps of 10 ms or 1 ms to 0.1 s: BDF2 across a step-size change).
+  steps of 10 ms or 1 ms to 0.1 s: BDF2 across a step-size change);
+- r100, r100_bdf1: data/R-100-step-1e3-tol-1e3-3.mat and
+  data/R-100-step-2e4-tol-1e3-3-bdf1.mat (the winding of
+  tutorials/run_ecore_hbfem.m: 300 turns, 100 ohm, 76 V; 23-point curve,
+  tolerances 1e-3; BDF2 at 1 ms for 60 steps, and TransientBDF1Solver at
+  0.2 ms for 300 steps).
 The file names give R, dt and the tolerance; the curve, the voltage and the
 switch-off time of the last two were identified by matching the recordings
 (each candidate among the tutorials' choices either reproduces a run to
@@ -37,7 +42,8 @@ OUT = Path(__file__).resolve().parent / "ecore_circuit_runs.npz"
 RUNS = {"dt1e-3": "V-1500-R-8-N-3000-step-1e3.mat", "dt5e-4": "V-1500-R-8-N-3000-step-5e4.mat",
         "r10": "R-10-step-1e3-tol-1e3-2.mat", "decay_r5": "shuaijian-R-5-step-1e3-tol-1e3-5.mat",
         "decay_r02_1e2": "shuaijian-R-0.2-step1-5e4-step2-1e-2-tol-1e3-5.mat",
-        "decay_r02_1e3": "shuaijian-R-0.2-step1-5e4-step2-1e-3-tol-1e3-5.mat"}
+        "decay_r02_1e3": "shuaijian-R-0.2-step1-5e4-step2-1e-3-tol-1e3-5.mat",
+        "r100": "R-100-step-1e3-tol-1e3-3.mat", "r100_bdf1": "R-100-step-2e4-tol-1e3-3-bdf1.mat"}
 
 
 def main():

@@ -136,37 +136,44 @@ def _ecore_setup():
     return d, tags, lib, w
 
 
-# recorded run: (steps, R, tolerance, B-H points, source switch-off time)
-RUNS = {"dt1e-3": ([1e-3] * 100, 8.0, 1e-4, 24, None),
-        "dt5e-4": ([5e-4] * 200, 8.0, 1e-4, 24, None),
-        "r10": ([1e-3] * 40, 10.0, 1e-3, 23, None),
-        "decay_r5": ([1e-3] * 100, 5.0, 1e-3, 23, 0.01),
-        "decay_r02_1e2": ([5e-4] * 20 + [1e-2] * 9, 0.2, 1e-3, 23, 0.01),
-        "decay_r02_1e3": ([5e-4] * 20 + [1e-3] * 90, 0.2, 1e-3, 23, 0.01)}
+# recorded run: (steps, R, tolerance, B-H points, source switch-off time,
+#                source amplitude, turns, BDF order)
+RUNS = {"dt1e-3": ([1e-3] * 100, 8.0, 1e-4, 24, None, 1500.0, 3000, 2),
+        "dt5e-4": ([5e-4] * 200, 8.0, 1e-4, 24, None, 1500.0, 3000, 2),
+        "r10": ([1e-3] * 40, 10.0, 1e-3, 23, None, 1500.0, 3000, 2),
+        "decay_r5": ([1e-3] * 100, 5.0, 1e-3, 23, 0.01, 1500.0, 3000, 2),
+        "decay_r02_1e2": ([5e-4] * 20 + [1e-2] * 9, 0.2, 1e-3, 23, 0.01, 1500.0, 3000, 2),
+        "decay_r02_1e3": ([5e-4] * 20 + [1e-3] * 90, 0.2, 1e-3, 23, 0.01, 1500.0, 3000, 2),
+        "r100": ([1e-3] * 60, 100.0, 1e-3, 23, None, 76.0, 300, 2),
+        "r100_bdf1": ([2e-4] * 300, 100.0, 1e-3, 23, None, 76.0, 300, 1)}
+RUN_BOUNDS = {"r10": 1e-5, "r100": 1e-3}
 
 
 @pytest.mark.parametrize("run", list(RUNS))
 def test_ecore_inrush_against_recorded_run(gpu, run):
     """The E-core inrush of tutorials/run_ecore_sdc_compare.m (1500 V, 3000
-    turns, BDF2) in the six recorded variants: R = 8 ohm at dt 1 ms and
-    0.5 ms (tolerances 1e-4, 0.1 s), R = 10 ohm (tolerances 1e-3, 40 ms),
-    R = 5 ohm with the source switched off at 10 ms (the current's decay
-    through the saturated core, 0.1 s), and R = 0.2 ohm with the switch-off
-    and a step change from 0.5 ms to 10 ms or 1 ms there; |B| at the origin
-    reaches 4.3 T and the current 47 A.  Current and probe |B| histories
-    against the recordings."""
-    from magpackage_amd.circuit import Circuit, TransientBDF2Solver
+    turns, BDF2) in six recorded variants: R = 8 ohm at dt 1 ms and 0.5 ms
+    (tolerances 1e-4, 0.1 s), R = 10 ohm (tolerances 1e-3, 40 ms), R = 5 ohm
+    with the source switched off at 10 ms (the current's decay through the
+    saturated core, 0.1 s), and R = 0.2 ohm with the switch-off and a step
+    change from 0.5 ms to 10 ms or 1 ms there; |B| at the origin reaches
+    4.3 T and the current 47 A.  Two more recordings drive the HBFEM
+    tutorial's winding (300 turns, 100 ohm, 76 V), with BDF2 and with BDF1.
+    Current and probe |B| histories against the recordings."""
+    from magpackage_amd.circuit import (Circuit, TransientBDF1Solver, TransientBDF2Solver,
+                                        Winding)
     from magpackage_amd.material import create_nonlinear
-    steps, R, tol, npts, t_off = RUNS[run]
-    d, tags, lib, w = _ecore_setup()
+    steps, R, tol, npts, t_off, amp, turns, order = RUNS[run]
+    d, tags, lib, w0 = _ecore_setup()
     lib[2] = create_nonlinear(B_DATA[:npts], H_DATA[:npts])
-    w.Resistance = R
-    circ = Circuit(R, 0.0, lambda t: 1500.0 * np.sin(2 * np.pi * 50 * t)
+    w = Winding("Primary", 50, turns, R, w0.CrossSectionArea)
+    w.setDirectionField(w0.DirectionField)
+    circ = Circuit(R, 0.0, lambda t: amp * np.sin(2 * np.pi * 50 * t)
                    * (1.0 if t_off is None or t < t_off else 0.0))
     ref = np.load(GOLDEN / "ecore_circuit_runs.npz")
     t_ref, i_ref, b_ref = ref[f"{run}_times"], ref[f"{run}_current"], ref[f"{run}_probeB"]
     asm, space, fixed = _engine(d["P"], d["T"], tags)
-    solver = TransientBDF2Solver(asm)
+    solver = (TransientBDF2Solver if order == 2 else TransientBDF1Solver)(asm)
     solver.Tolerance = tol
     solver.RelTolerance = tol
     sig = {t: 0.0 for t in (1, 2, 50, 60)}
@@ -188,11 +195,14 @@ def test_ecore_inrush_against_recorded_run(gpu, run):
     print(f"max |dI| / max|I| = {ei:.2e}, max |dB| / max|B| = {eb:.2e}; first steps "
           f"I {cur[:3]} vs {i_ref[:3]}; Newton total {sum(solver.NewtonIterations)}, "
           f"PCG total {sum(solver.PcgIterations)}", flush=True)
-    # measured: 4e-9 on the 1e-4-tolerance runs; 2e-6 on the 1e-3-tolerance
-    # ones, whose Newton stops are looser (a residual near the threshold can
-    # stop one iteration apart)
-    bound = 1e-7 if tol <= 1e-4 else 1e-5
-    assert np.abs(cur[:2] - i_ref[:2]).max() <= 1e-8 * np.abs(i_ref[:2]).max()
+    # bounds over the measured deviations: ~1e-9 where the recording's Newton
+    # tolerance is 1e-4 or the steps converge well below it; 2e-6 (r10) and
+    # 4e-4 (r100) where two Newton iterations per step stop at 1e-3, so a
+    # residual near the threshold stops one iteration apart.  In the
+    # 100-ohm runs ~97% of the source voltage is induced, so flux errors
+    # reach the current amplified ~30 times.
+    bound = RUN_BOUNDS.get(run, 1e-7)
+    assert np.abs(cur[:2] - i_ref[:2]).max() <= 1e-7 * np.abs(i_ref[:2]).max()
     assert ei <= bound and eb <= bound
 
 
