@@ -14,6 +14,7 @@
 // iterations.  Dot products reduce per block into a fixed slot array and the
 // final sum runs in slot order in one block, so results are deterministic.
 #include <cmath>
+#include <cstdlib>
 
 #include "mpk_common.hpp"
 
@@ -628,6 +629,84 @@ int grid_parts(int64_t n, int per_block) {
     return (int)g;
 }
 
+// Solver stream: a solve runs on a private stream of this thread and device,
+// ordered after the caller's stream by an event at the start and before it by
+// one at the end.  Its iteration batches are captured once into a HIP graph
+// and replayed: on small systems an iteration is a few microsecond-long
+// kernels, and one graph launch per batch replaces their individual launches.
+// The captured kernels take fixed device pointers (the solver's workspace), so
+// every replay runs the same iterations.  MPK_SOLVER_GRAPHS=0 launches the
+// kernels one by one (same results).
+SideStream* thread_solver_stream(int dev) {
+    // created once per thread and device and kept to the end of the process
+    static thread_local SideStream* streams[64] = {};
+    if (dev < 0 || dev >= 64) return nullptr;
+    if (!streams[dev]) streams[dev] = new SideStream();
+    return streams[dev];
+}
+
+struct SolveStream {
+    hipStream_t caller = nullptr, s = nullptr;
+    SideStream* side = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graphs = true;
+    hipError_t err = hipSuccess;
+    explicit SolveStream(hipStream_t c) : caller(c) {
+        int dev = 0;
+        err = hipGetDevice(&dev);
+        if (err != hipSuccess) return;
+        side = thread_solver_stream(dev);
+        if (!side) {
+            err = hipErrorInvalidDevice;
+            return;
+        }
+        err = side->ensure();
+        if (err != hipSuccess) return;
+        s = side->s;
+        err = hipEventRecord(side->fork, caller);
+        if (err == hipSuccess) err = hipStreamWaitEvent(s, side->fork, 0);
+        const char* e = std::getenv("MPK_SOLVER_GRAPHS");
+        graphs = !(e && e[0] == '0');
+    }
+    ~SolveStream() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (side && s) {
+            (void)hipEventRecord(side->join, s);
+            (void)hipStreamWaitEvent(caller, side->join, 0);
+        }
+    }
+};
+
+// Launches `batch` iterations (body() launches one) on the solve stream: a
+// full batch (`full` iterations) as a replay of the graph captured from the
+// first full batch, a shorter one directly.  A capture that fails falls back
+// to direct launches for the rest of the solve.
+template <class F>
+hipError_t run_batch(SolveStream& ss, int batch, int full, F&& body) {
+    if (batch == full && ss.graphs) {
+        if (!ss.exec) {
+            hipError_t e = hipStreamBeginCapture(ss.s, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                for (int k = 0; k < batch; ++k) body();
+                hipGraph_t g = nullptr;
+                e = hipStreamEndCapture(ss.s, &g);
+                if (e == hipSuccess) e = hipGraphInstantiate(&ss.exec, g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                ss.graphs = false;
+                ss.exec = nullptr;
+                for (int k = 0; k < batch; ++k) body();
+                return hipGetLastError();
+            }
+        }
+        return hipGraphLaunch(ss.exec, ss.s);
+    }
+    for (int k = 0; k < batch; ++k) body();
+    return hipGetLastError();
+}
+
 }  // namespace
 }  // namespace mpk
 
@@ -674,7 +753,9 @@ int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* va
     if (n == 0) return MPK_OK;
     MPK_REQUIRE(colptr && rowind && vals && b && x && work, MPK_ERR_ARGS, "NULL argument");
     if (check_every < 1) check_every = 1;
-    hipStream_t st = as_stream(stream);
+    SolveStream ss(as_stream(stream));
+    MPK_HIP_TRY(ss.err);
+    hipStream_t st = ss.s;
     // work layout: dinv, r, z, p, q (n each), partials (3 * kMaxPartials), scal (8)
     double* dinv = work;
     double* r = dinv + n;
@@ -700,7 +781,7 @@ int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* va
     double rel = 1.0;
     while (it < max_iter) {
         const int batch = (max_iter - it) < check_every ? (max_iter - it) : check_every;
-        for (int k = 0; k < batch; ++k) {
+        MPK_HIP_TRY(run_batch(ss, batch, check_every, [&]() {
             hipLaunchKernelGGL(k_spmv_dot, dim3(g_row), dim3(kB), 0, st, colptr, rowind, vals,
                                fixed, n, p, q, pa);
             hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, pa, g_row, scal);
@@ -708,8 +789,7 @@ int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* va
                                z, pb, pc);
             hipLaunchKernelGGL(k_beta, dim3(1), dim3(kB), 0, st, pb, pc, g_vec, scal);
             hipLaunchKernelGGL(k_direction, dim3(g_vec), dim3(kB), 0, st, n, scal, z, p);
-        }
-        MPK_HIP_TRY(hipGetLastError());
+        }));
         it += batch;
         MPK_HIP_TRY(hipMemcpyAsync(h, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
         MPK_HIP_TRY(hipStreamSynchronize(st));
@@ -738,7 +818,9 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
                                  aux->ln_idx && aux->ln_vals)),
                 MPK_ERR_ARGS, "incomplete auxiliary space");
     if (check_every < 1) check_every = 1;
-    hipStream_t st = as_stream(stream);
+    SolveStream ss(as_stream(stream));
+    MPK_HIP_TRY(ss.err);
+    hipStream_t st = ss.s;
     // work: dinv, r, z, p, q (n each), dninv, g, phi0, phi1 (nv each), partials, scal
     double* dinv = work;
     double* r = dinv + n;
@@ -800,7 +882,7 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
     double rel = 1.0;
     while (it < max_iter) {
         const int batch = (max_iter - it) < check_every ? (max_iter - it) : check_every;
-        for (int k = 0; k < batch; ++k) {
+        MPK_HIP_TRY(run_batch(ss, batch, check_every, [&]() {
             hipLaunchKernelGGL(k_spmv_dot, dim3(g_row), dim3(kB), 0, st, colptr, rowind, vals,
                                fixed, n, p, q, pa);
             hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, pa, g_row, scal);
@@ -808,8 +890,7 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
             precondition();
             hipLaunchKernelGGL(k_beta, dim3(1), dim3(kB), 0, st, pb, pc, g_vec, scal);
             hipLaunchKernelGGL(k_direction, dim3(g_vec), dim3(kB), 0, st, n, scal, z, p);
-        }
-        MPK_HIP_TRY(hipGetLastError());
+        }));
         it += batch;
         MPK_HIP_TRY(hipMemcpyAsync(h, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
         MPK_HIP_TRY(hipStreamSynchronize(st));
@@ -847,7 +928,9 @@ int mpk_cocg_aux_solve(const int32_t* colptr, const int32_t* rowind, const doubl
                                       aux->ln_idx && aux->ln_vals))),
                 MPK_ERR_ARGS, "incomplete auxiliary space");
     if (check_every < 1) check_every = 1;
-    hipStream_t st = as_stream(stream);
+    SolveStream ss(as_stream(stream));
+    MPK_HIP_TRY(ss.err);
+    hipStream_t st = ss.s;
     // work: dinv (n, padded to even), r, z, p, q (2n each), dninv (nv, padded to even),
     // g, phi0, phi1 (2nv each), partials (3 x kMaxPartials), scal (16)
     double* dinv = work;
@@ -917,7 +1000,7 @@ int mpk_cocg_aux_solve(const int32_t* colptr, const int32_t* rowind, const doubl
     double rel = 1.0;
     while (it < max_iter) {
         const int batch = (max_iter - it) < check_every ? (max_iter - it) : check_every;
-        for (int k = 0; k < batch; ++k) {
+        MPK_HIP_TRY(run_batch(ss, batch, check_every, [&]() {
             hipLaunchKernelGGL(k_c_spmv_dot, dim3(g_row), dim3(kB), 0, st, colptr, rowind,
                                vals_re, vals_im, fixed, n, p, q, pa, pb);
             hipLaunchKernelGGL(k_c_alpha, dim3(1), dim3(kB), 0, st, pa, pb, g_row, scal);
@@ -926,8 +1009,7 @@ int mpk_cocg_aux_solve(const int32_t* colptr, const int32_t* rowind, const doubl
             precondition();
             hipLaunchKernelGGL(k_c_beta, dim3(1), dim3(kB), 0, st, pa, pb, pc, g_vec, 0, scal);
             hipLaunchKernelGGL(k_c_direction, dim3(g_vec), dim3(kB), 0, st, n, scal, z, p);
-        }
-        MPK_HIP_TRY(hipGetLastError());
+        }));
         it += batch;
         MPK_HIP_TRY(hipMemcpyAsync(h, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
         MPK_HIP_TRY(hipStreamSynchronize(st));
