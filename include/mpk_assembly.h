@@ -339,7 +339,12 @@ int mpk_pcg_work_size(int64_t n, int64_t* size_out);
  * (src/BoundaryCondition.m:81-112): fixed [n] (uint8, NULL = none) removes the
  * rows and columns of fixed DOFs and puts a unit diagonal back; b's fixed
  * entries are treated as 0.  Starts from x = 0; stops when |r|/|b| <= rel_tol
- * (checked every check_every iterations) or after max_iter.  Synchronises. */
+ * (checked every check_every iterations) or after max_iter.  Synchronises.
+ * The iterations run on an internal stream of the calling thread, ordered
+ * after `stream` and joined back into it; each batch of check_every
+ * iterations is captured once into a HIP graph and replayed
+ * (MPK_SOLVER_GRAPHS=0: plain launches, same results).  The same holds for
+ * mpk_pcg_aux_solve and mpk_cocg_aux_solve. */
 int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* vals, int64_t n,
                   const uint8_t* fixed, const double* b, double* x, double* work,
                   double rel_tol, int max_iter, int check_every, int* iters_out,
