@@ -23,7 +23,6 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .fem_tables import get_quadrature_data, matlab_flat, nedelec_tet_p1
 from .material import pack_material_data
 
 

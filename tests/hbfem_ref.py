@@ -12,15 +12,13 @@ test follow the reference. Test infrastructure only.
 """
 from __future__ import annotations
 
-import math
-
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from oracle import oracle as O
 from magpackage_amd.material import pack_material_data
-from tests.helpers import stiffness_tables
+from tests.helpers import mass_tables, stiffness_tables
 
 
 def _kernel(mi, tags, X, aft, packed, request_j):
@@ -60,7 +58,6 @@ def hbfem_coupled_cpu(mesh, lib, aft, winding, Rc, V_harm, fixed, kadd, tol=1e-4
     w = 2 * np.pi * aft.BaseFreq * aft.Harmonics
     packed = pack_material_data(lib)
     tags = mesh.RegionTags.astype(np.float64)
-    from tests.helpers import mass_tables
     qm, val, _ = mass_tables(2)
     Iw, Vw = O.winding(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, winding["region"],
                        winding["dir"], np.zeros(3), winding["turns"] / winding["area"], qm, val)

@@ -15,7 +15,6 @@ independent of the restatement's saddle point."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
-import scipy.sparse.linalg as spla
 
 from oracle import oracle as O
 from magpackage_amd.material import create_linear

@@ -6,7 +6,6 @@ from pathlib import Path
 from types import SimpleNamespace
 
 import numpy as np
-import pytest
 
 from magpackage_amd.coil import CoilGeometryUtils as CG
 from magpackage_amd.meshgen import kuhn_box
