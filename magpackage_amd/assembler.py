@@ -120,10 +120,13 @@ class DofHandler:
         self.EntityMaps = {}
 
     def distributeDofs(self, space: FunctionSpace, targetRegionTags=None):
-        """DofHandler.distributeDofs, Nedelec branch (src/DofHandler.m:72-105)."""
+        """DofHandler.distributeDofs (src/DofHandler.m:23-122): Nedelec P1
+        (:72-105) and Lagrange P1 (:52-70) spaces."""
         key = space.toString()
         if key in self.DofMaps:
             return self.DofMaps[key], self.SpaceOffsets[key]
+        if space.Type.lower() == "lagrange" and space.Order == 1:
+            return self._distribute_nodal(key, targetRegionTags)
         if space.Type.lower() != "nedelec" or space.Order != 1:
             raise ValueError(f"Unsupported space: {key} (the engine covers Nedelec P1)")
         mesh = self.Mesh
@@ -149,6 +152,31 @@ class DofHandler:
         return cell_dofs, offset
 
     distribute_dofs = distributeDofs
+
+    def _distribute_nodal(self, key, targetRegionTags):
+        """Lagrange P1: the nodes of the active elements, in increasing node id,
+        numbered offset + 1, offset + 2, ...; CellDofs (Ne, 4) int32, 0 at
+        inactive nodes (src/DofHandler.m:52-70)."""
+        mesh = self.Mesh
+        ne, nn = mesh.NumElements, mesh.NumNodes
+        dev = _dev()
+        T0 = mesh.T_dev.view(ne, 4).to(torch.int64) - 1
+        if targetRegionTags is not None and len(np.atleast_1d(targetRegionTags)) > 0:
+            act = torch.from_numpy(np.isin(mesh.RegionTags, np.atleast_1d(targetRegionTags)))
+            T0a = T0[act.to(dev)]
+        else:
+            T0a = T0
+        nodes = torch.unique(T0a)  # sorted, as MATLAB's unique
+        offset = self.NumGlobalDofs
+        emap = torch.zeros(max(nn, 1), dtype=torch.int32, device=dev)
+        emap[nodes] = (offset + 1 + torch.arange(nodes.numel(), device=dev)).to(torch.int32)
+        cell = emap[T0].contiguous()
+        self.DofMaps[key] = cell
+        self.SpaceOffsets[key] = offset
+        self.SpaceLocalSizes[key] = int(nodes.numel())
+        self.EntityMaps[key] = emap[:nn]
+        self.NumGlobalDofs += int(nodes.numel())
+        return cell, offset
 
     def packForKernel(self, space: FunctionSpace) -> dict:
         key = space.toString()
@@ -525,6 +553,41 @@ class Assembler:
 
     def source_quadrature(self) -> L.Quadrature:
         return L.standard_quadrature(max(self.Config.DefaultQuadratureOrder, 2))
+
+    def assembleScalarLaplacian(self, space, coeffMap) -> CscMatrix:
+        """K = assembleScalarLaplacian(space, coeffMap) (src/Assembler.m:121-132):
+        assemble_scalar_laplacian_kernel_mex at the 1-point rule over the
+        space's node DOFs, then sparse(I(valid), J(valid), V(valid), N, N),
+        N = NumGlobalDofs, as a device CSC matrix."""
+        from .fem_tables import get_quadrature_data, lagrange_tet_p1, matlab_flat
+        if space.Type.lower() != "lagrange":
+            raise ValueError("assembleScalarLaplacian needs a Lagrange space")
+        mesh = self.Mesh
+        cell = self.DofHandler.DofMaps[space.toString()]
+        n = self.DofHandler.NumGlobalDofs
+        ne, nn = mesh.NumElements, mesh.NumNodes
+        dev = _dev()
+        q_pts, q_w = get_quadrature_data("tet", 1)
+        grad = np.ascontiguousarray(matlab_flat(lagrange_tet_p1(q_pts)[1]))
+        q = L.make_quadrature(q_w)
+        coeff = torch.from_numpy(expand_material_map(coeffMap, mesh.RegionTags)).to(dev)
+        Td = mesh.T_dev.to(torch.float64)
+        Dd = cell.reshape(-1).to(torch.float64)
+        nt = 16 * ne
+        Io = torch.empty(max(nt, 1), dtype=torch.float64, device=dev)
+        Jo, Vo = torch.empty_like(Io), torch.empty_like(Io)
+        L.check(L.lib.mpk_scalar_laplacian_coo(
+            L.ptr(mesh.P_dev), nn, L.ptr(Td), ne, L.ptr(Dd), L.ptr(coeff), C.byref(q),
+            grad.ctypes.data_as(C.c_void_p), L.ptr(Io), L.ptr(Jo), L.ptr(Vo), L.stream_ptr()))
+        colptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        rowind = torch.empty(max(nt, 1), dtype=torch.int32, device=dev)
+        vals = torch.empty(max(nt, 1), dtype=torch.float64, device=dev)
+        nnz = C.c_int64(0)
+        L.check(L.lib.mpk_sparse_coo_to_csc(L.ptr(Io), L.ptr(Jo), L.ptr(Vo), nt, n, n, 1,
+                                            L.ptr(colptr), L.ptr(rowind), L.ptr(vals), nt,
+                                            C.byref(nnz), L.stream_ptr()))
+        k = int(nnz.value)
+        return CscMatrix((n, n), colptr, rowind[:k], vals[:k])
 
     def assembleSource(self, space, sourceMap, out: torch.Tensor | None = None):
         """F = assembleSource(space, sourceMap) (src/Assembler.m:78-97): the
