@@ -17,7 +17,17 @@ probe |B| at the origin):
   data/R-100-step-2e4-tol-1e3-3-bdf1.mat (the winding of
   tutorials/run_ecore_hbfem.m: 300 turns, 100 ohm, 76 V; 23-point curve,
   tolerances 1e-3; BDF2 at 1 ms for 60 steps, and TransientBDF1Solver at
-  0.2 ms for 300 steps).
+  0.2 ms for 300 steps);
+- r10_dt1e-4: data/R-10-step-1e4-tol-1e3-2.mat (r10 at dt = 0.1 ms, 400 steps);
+- r100_dt2e-4, r100_dt5e-4: data/R-100-step-{1e4,5e4}-tol-1e3-3.mat (r100 with
+  BDF2 at 0.2 ms for 300 steps and 0.5 ms for 120 steps; the first file's name
+  says 1e4, its time vector 0.2 ms);
+- decay_r02, decay_r02_5e-4, decay_r5_5e-4: data/shuaijian-R-0.2-step-{1e3,5e4}-
+  tol-1e3-5.mat and data/shuaijian-R-5-step-5e4-tol-1e3-5.mat (the switch-off
+  runs at a constant dt of 1 ms or 0.5 ms);
+- r02_long, r02_long_tol1e-2: data/R-0.2-step_1e3_tol_1e{3,2}_100.mat
+  (R = 0.2 ohm, 23-point curve, no switch-off, 2000 steps of 1 ms, Newton
+  tolerances 1e-3 and 1e-2).
 The file names give R, dt and the tolerance; the curve, the voltage and the
 switch-off time of the last two were identified by matching the recordings
 (each candidate among the tutorials' choices either reproduces a run to
@@ -43,7 +53,15 @@ RUNS = {"dt1e-3": "V-1500-R-8-N-3000-step-1e3.mat", "dt5e-4": "V-1500-R-8-N-3000
         "r10": "R-10-step-1e3-tol-1e3-2.mat", "decay_r5": "shuaijian-R-5-step-1e3-tol-1e3-5.mat",
         "decay_r02_1e2": "shuaijian-R-0.2-step1-5e4-step2-1e-2-tol-1e3-5.mat",
         "decay_r02_1e3": "shuaijian-R-0.2-step1-5e4-step2-1e-3-tol-1e3-5.mat",
-        "r100": "R-100-step-1e3-tol-1e3-3.mat", "r100_bdf1": "R-100-step-2e4-tol-1e3-3-bdf1.mat"}
+        "r100": "R-100-step-1e3-tol-1e3-3.mat", "r100_bdf1": "R-100-step-2e4-tol-1e3-3-bdf1.mat",
+        "r10_dt1e-4": "R-10-step-1e4-tol-1e3-2.mat",
+        "r100_dt2e-4": "R-100-step-1e4-tol-1e3-3.mat",
+        "r100_dt5e-4": "R-100-step-5e4-tol-1e3-3.mat",
+        "decay_r02": "shuaijian-R-0.2-step-1e3-tol-1e3-5.mat",
+        "decay_r02_5e-4": "shuaijian-R-0.2-step-5e4-tol-1e3-5.mat",
+        "decay_r5_5e-4": "shuaijian-R-5-step-5e4-tol-1e3-5.mat",
+        "r02_long": "R-0.2-step_1e3_tol_1e3_100.mat",
+        "r02_long_tol1e-2": "R-0.2-step_1e3_tol_1e2_100.mat"}
 
 
 def main():
@@ -51,8 +69,10 @@ def main():
     for key, name in RUNS.items():
         path = REF / name
         d = sio.loadmat(str(path), squeeze_me=True, struct_as_record=False)
-        s = d["info_bdf"]
-        out[f"{key}_times"] = np.asarray(s.times, dtype=np.float64).ravel()
+        s = d["info_bdf"] if "info_bdf" in d else d["info"]
+        tname = next(f for f in ("times", "timeSteps_bdf", "times_bdf", "timeVec")
+                     if f in s._fieldnames)
+        out[f"{key}_times"] = np.asarray(getattr(s, tname), dtype=np.float64).ravel()
         out[f"{key}_current"] = np.asarray(s.CurrentHistory, dtype=np.float64).ravel()
         out[f"{key}_probeB"] = np.asarray(s.ProbeB_History, dtype=np.float64).ravel()
         out[f"{key}_sha256"] = np.array(hashlib.sha256(path.read_bytes()).hexdigest())

@@ -145,7 +145,16 @@ RUNS = {"dt1e-3": ([1e-3] * 100, 8.0, 1e-4, 24, None, 1500.0, 3000, 2),
         "decay_r02_1e2": ([5e-4] * 20 + [1e-2] * 9, 0.2, 1e-3, 23, 0.01, 1500.0, 3000, 2),
         "decay_r02_1e3": ([5e-4] * 20 + [1e-3] * 90, 0.2, 1e-3, 23, 0.01, 1500.0, 3000, 2),
         "r100": ([1e-3] * 60, 100.0, 1e-3, 23, None, 76.0, 300, 2),
-        "r100_bdf1": ([2e-4] * 300, 100.0, 1e-3, 23, None, 76.0, 300, 1)}
+        "r100_bdf1": ([2e-4] * 300, 100.0, 1e-3, 23, None, 76.0, 300, 1),
+        "r10_dt1e-4": ([1e-4] * 400, 10.0, 1e-3, 23, None, 1500.0, 3000, 2),
+        "r100_dt2e-4": ([2e-4] * 300, 100.0, 1e-3, 23, None, 76.0, 300, 2),
+        "r100_dt5e-4": ([5e-4] * 120, 100.0, 1e-3, 23, None, 76.0, 300, 2),
+        "decay_r02": ([1e-3] * 100, 0.2, 1e-3, 23, 0.01, 1500.0, 3000, 2),
+        "decay_r02_5e-4": ([5e-4] * 200, 0.2, 1e-3, 23, 0.01, 1500.0, 3000, 2),
+        "decay_r5_5e-4": ([5e-4] * 200, 5.0, 1e-3, 23, 0.01, 1500.0, 3000, 2),
+        # the first 200 of 2000 recorded steps
+        "r02_long": ([1e-3] * 200, 0.2, 1e-3, 23, None, 1500.0, 3000, 2),
+        "r02_long_tol1e-2": ([1e-3] * 100, 0.2, 1e-2, 23, None, 1500.0, 3000, 2)}
 RUN_BOUNDS = {"r10": 1e-5, "r100": 1e-3}
 
 
@@ -159,6 +168,11 @@ def test_ecore_inrush_against_recorded_run(gpu, run):
     change from 0.5 ms to 10 ms or 1 ms there; |B| at the origin reaches
     4.3 T and the current 47 A.  Two more recordings drive the HBFEM
     tutorial's winding (300 turns, 100 ohm, 76 V), with BDF2 and with BDF1.
+    Six more repeat these set-ups at other step sizes: R = 10 ohm at 0.1 ms,
+    the 100-ohm winding with BDF2 at 0.2 ms and 0.5 ms, and the switch-off
+    runs (0.2 ohm at 1 ms and 0.5 ms, 5 ohm at 0.5 ms). Two 2000-step runs at
+    0.2 ohm without the switch-off (Newton tolerances 1e-3 and 1e-2) are
+    compared over their first 200 and 100 steps.
     Current and probe |B| histories against the recordings."""
     from magpackage_amd.circuit import (Circuit, TransientBDF1Solver, TransientBDF2Solver,
                                         Winding)
@@ -171,7 +185,8 @@ def test_ecore_inrush_against_recorded_run(gpu, run):
     circ = Circuit(R, 0.0, lambda t: amp * np.sin(2 * np.pi * 50 * t)
                    * (1.0 if t_off is None or t < t_off else 0.0))
     ref = np.load(GOLDEN / "ecore_circuit_runs.npz")
-    t_ref, i_ref, b_ref = ref[f"{run}_times"], ref[f"{run}_current"], ref[f"{run}_probeB"]
+    nst = len(steps)
+    t_ref, i_ref, b_ref = (ref[f"{run}_{f}"][:nst] for f in ("times", "current", "probeB"))
     asm, space, fixed = _engine(d["P"], d["T"], tags)
     solver = (TransientBDF2Solver if order == 2 else TransientBDF1Solver)(asm)
     solver.Tolerance = tol
