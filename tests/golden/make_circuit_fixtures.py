@@ -27,7 +27,8 @@ probe |B| at the origin):
   runs at a constant dt of 1 ms or 0.5 ms);
 - r02_long, r02_long_tol1e-2: data/R-0.2-step_1e3_tol_1e{3,2}_100.mat
   (R = 0.2 ohm, 23-point curve, no switch-off, 2000 steps of 1 ms, Newton
-  tolerances 1e-3 and 1e-2).
+  tolerances 1e-3 and 1e-2); r02_long_dt1e-4: data/R-0.2-step_1e4_tol_1e3_100.mat
+  (the same at 0.1 ms, 20000 steps; only the first 1000 are stored).
 The file names give R, dt and the tolerance; the curve, the voltage and the
 switch-off time of the last two were identified by matching the recordings
 (each candidate among the tutorials' choices either reproduces a run to
@@ -61,7 +62,8 @@ RUNS = {"dt1e-3": "V-1500-R-8-N-3000-step-1e3.mat", "dt5e-4": "V-1500-R-8-N-3000
         "decay_r02_5e-4": "shuaijian-R-0.2-step-5e4-tol-1e3-5.mat",
         "decay_r5_5e-4": "shuaijian-R-5-step-5e4-tol-1e3-5.mat",
         "r02_long": "R-0.2-step_1e3_tol_1e3_100.mat",
-        "r02_long_tol1e-2": "R-0.2-step_1e3_tol_1e2_100.mat"}
+        "r02_long_tol1e-2": "R-0.2-step_1e3_tol_1e2_100.mat",
+        "r02_long_dt1e-4": "R-0.2-step_1e4_tol_1e3_100.mat"}
 
 
 def main():
@@ -75,6 +77,9 @@ def main():
         out[f"{key}_times"] = np.asarray(getattr(s, tname), dtype=np.float64).ravel()
         out[f"{key}_current"] = np.asarray(s.CurrentHistory, dtype=np.float64).ravel()
         out[f"{key}_probeB"] = np.asarray(s.ProbeB_History, dtype=np.float64).ravel()
+        if key == "r02_long_dt1e-4":
+            for f in ("times", "current", "probeB"):
+                out[f"{key}_{f}"] = out[f"{key}_{f}"][:1000]
         out[f"{key}_sha256"] = np.array(hashlib.sha256(path.read_bytes()).hexdigest())
         assert out[f"{key}_times"].size == out[f"{key}_current"].size
     np.savez_compressed(OUT, **out)

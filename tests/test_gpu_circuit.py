@@ -154,7 +154,9 @@ RUNS = {"dt1e-3": ([1e-3] * 100, 8.0, 1e-4, 24, None, 1500.0, 3000, 2),
         "decay_r5_5e-4": ([5e-4] * 200, 5.0, 1e-3, 23, 0.01, 1500.0, 3000, 2),
         # the first 200 of 2000 recorded steps
         "r02_long": ([1e-3] * 200, 0.2, 1e-3, 23, None, 1500.0, 3000, 2),
-        "r02_long_tol1e-2": ([1e-3] * 100, 0.2, 1e-2, 23, None, 1500.0, 3000, 2)}
+        "r02_long_tol1e-2": ([1e-3] * 100, 0.2, 1e-2, 23, None, 1500.0, 3000, 2),
+        # the first 300 of 20000 recorded steps
+        "r02_long_dt1e-4": ([1e-4] * 300, 0.2, 1e-3, 23, None, 1500.0, 3000, 2)}
 RUN_BOUNDS = {"r10": 1e-5, "r100": 1e-3}
 
 
@@ -172,7 +174,8 @@ def test_ecore_inrush_against_recorded_run(gpu, run):
     the 100-ohm winding with BDF2 at 0.2 ms and 0.5 ms, and the switch-off
     runs (0.2 ohm at 1 ms and 0.5 ms, 5 ohm at 0.5 ms). Two 2000-step runs at
     0.2 ohm without the switch-off (Newton tolerances 1e-3 and 1e-2) are
-    compared over their first 200 and 100 steps.
+    compared over their first 200 and 100 steps, and a 20000-step run of it
+    at 0.1 ms over its first 300.
     Current and probe |B| histories against the recordings."""
     from magpackage_amd.circuit import (Circuit, TransientBDF1Solver, TransientBDF2Solver,
                                         Winding)
