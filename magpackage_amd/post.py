@@ -8,6 +8,16 @@ field:
 - ``probeB(A_sol, point)`` (:403-441): B in the tet that holds the point, or 0
   outside the mesh.
 - ``probeLine(A_sol, start, end, numPts, "B")`` (:505-532).
+- ``computeMagneticEnergy(A_sol, MatLibData)`` (:544-611, :691-760): with
+  linear materials only, 0.5 Re(A^H K A) with K assembled on the GPU. Otherwise
+  it sums over elements the volume times the energy density at the element's B.
+  That density is 0.5 nu B^2 for linear materials, and Simpson's rule for the
+  integral of H dB over [0, |B|] with H = nu(b^2) b for nonlinear ones. The
+  reference sums it over the 4 quadrature points of a constant B, with weights
+  adding up to the volume. nu comes from the engine's B-H evaluation
+  (mpk_eval_nu, the MEX's evaluate_nu_derivative). MaterialLib.evaluate,
+  which the reference calls here, resets nu to nu0 below 1e-6 rather than
+  below 1e-7: a B-H table never reaches that range.
 - ``probeSolution(A_sol, point)`` (:443-454, :612-635): A interpolated with the
   edge basis at the point. The reference's element's basis values are mapped by
   the covariant Piola map, x_ref = J^-1 (x - p1) and N = N_ref J^-1.
@@ -140,6 +150,49 @@ class PostProcessor:
             return np.zeros((3, ncols)).squeeze(), float("nan")
         B = self.computeElementB(a)[:, e, :].cpu().numpy()
         return (B[:, 0] if ncols == 1 else B), e + 1
+
+    def computeMagneticEnergy(self, A_sol, MatLibData) -> float:
+        """Magnetic energy [J] of the field A_sol (real or complex)."""
+        import ctypes as C
+        from . import _lib as L
+        mats = dict(MatLibData)
+        space = FunctionSpace("Nedelec", 1)
+        a = A_sol if torch.is_tensor(A_sol) else torch.from_numpy(np.asarray(A_sol))
+        dev = self.Mesh.P_dev.device
+        a = a.to(dev)
+        if all(m.Type == "Linear" for m in mats.values()):
+            K = self.Assembler.assembleStiffness(space, {t: m.Nu_Linear for t, m in mats.items()})
+            colptr = K.colptr.to(torch.int32).contiguous()
+            n = K.shape[0]
+            parts = [a.real, a.imag] if a.is_complex() else [a]
+            energy = 0.0
+            for v in parts:
+                x = v.to(torch.float64).contiguous()
+                y = torch.empty(n, dtype=torch.float64, device=dev)
+                L.check(L.lib.mpk_spmv_sym(L.ptr(colptr), L.ptr(K.rowind), L.ptr(K.values), n,
+                                           L.ptr(x), L.ptr(y), L.stream_ptr()))
+                energy += 0.5 * float(x @ y)   # Re(A^H K A) for symmetric real K
+            return energy
+        B = self.computeElementB(a)[:, :, 0]
+        bmag = torch.sqrt((B.abs() ** 2).sum(dim=0)).to(torch.float64).contiguous()
+        ne = bmag.numel()
+        mat, keep = self.Assembler.material_tables(mats)
+        tags = self.Mesh.Tags_dev
+        nu_half = torch.empty(ne, dtype=torch.float64, device=dev)
+        nu_full = torch.empty_like(nu_half)
+        for s_, out in (((0.5 * bmag) ** 2, nu_half), (bmag ** 2, nu_full)):
+            s_ = s_.contiguous()
+            L.check(L.lib.mpk_eval_nu(L.ptr(s_), L.ptr(tags), ne, C.byref(mat), L.ptr(out),
+                                      None, L.stream_ptr()))
+        lin = np.zeros(int(self.Mesh.RegionTags.max()) + 1, dtype=bool)
+        for t, m in mats.items():
+            if int(t) < lin.size and m.Type == "Linear":
+                lin[int(t)] = True
+        is_lin = torch.from_numpy(lin[self.Mesh.RegionTags]).to(dev)
+        simpson = (bmag / 6.0) * (4.0 * nu_half * (0.5 * bmag) + nu_full * bmag)
+        dens = torch.where(is_lin, 0.5 * nu_full * bmag ** 2, simpson)
+        vol = torch.from_numpy(1.0 / np.abs(np.linalg.det(self._Jinv)) / 6.0).to(dev)
+        return float((vol * dens).sum())
 
     def probeLine(self, A_sol, startPt, endPt, numPts: int, type_: str = "B") -> np.ndarray:
         """(numPts, 3) samples on the segment (linspace per coordinate)."""
