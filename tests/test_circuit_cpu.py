@@ -14,57 +14,15 @@ L is computed here by a gauge-free route: a penalised curl-curl solve,
 independent of the restatement's saddle point."""
 import numpy as np
 import pytest
-import scipy.sparse as sp
 
-from oracle import oracle as O
 from magpackage_amd.material import create_linear
 from magpackage_amd.meshgen import ecore_box
-from tests.circuit_ref import boundary_nodes, circuit_bdf_cpu
-from tests.helpers import mass_tables, stiffness_tables
+from tests.circuit_ref import circuit_bdf_cpu, inductance_matrix
 
 
 def _inductance(km, nu, region, turns, area, direction):
-    """L = W^T A for K A = W with the gauge G^T M A = 0, solved as
-    (K + gamma M G Ln^-1 G^T M) A = W~ on the free edges (SPD; equals K on
-    gauged fields)."""
-    mi = O.mesh_inputs(km)
-    n = mi["ndof"]
-    P, T, D, S = mi["P"], mi["T"], mi["Dofs"], mi["Signs"]
-    qw, curl, _ = stiffness_tables()
-    qm, val, _ = mass_tables(2)
-    K = O.sparse_scipy(*O.curl_curl(P, T, D, S, nu, qw, curl), n, n, drop_zeros=False)
-    M = O.sparse_scipy(*O.mass(P, T, D, S, None, qm, val), n, n, drop_zeros=False)
-    Iw, Vw = O.winding(P, T, D, S, km.RegionTags.astype(np.float64), region, direction,
-                       np.zeros(3), turns / area, qm, val)
-    W = np.zeros(n)
-    ok = Iw > 0
-    np.add.at(W, Iw[ok].astype(np.int64) - 1, Vw[ok])
-    edges = mi["edges"]
-    emap = np.zeros(edges.shape[0], dtype=np.int64)
-    emap[mi["t2e"].ravel() - 1] = mi["cell"].ravel()
-    fixed = np.zeros(n, dtype=bool)
-    fd = emap[O.boundary_edges(km.T, edges)]
-    fixed[fd[fd > 0] - 1] = True
-    free_nodes = np.setdiff1d(np.arange(km.P.shape[0]), boundary_nodes(km.T))
-    fid = np.full(km.P.shape[0], -1)
-    fid[free_nodes] = np.arange(free_nodes.size)
-    rows = emap - 1
-    a, b = fid[edges[:, 0] - 1], fid[edges[:, 1] - 1]
-    ma, mb = (a >= 0) & (rows >= 0), (b >= 0) & (rows >= 0)
-    G = sp.csr_matrix((np.r_[-np.ones(ma.sum()), np.ones(mb.sum())],
-                       (np.r_[rows[ma], rows[mb]], np.r_[a[ma], b[mb]])),
-                      shape=(n, free_nodes.size))
-    f = ~fixed
-    Kf, Mf, Gf, Wf = K[f][:, f], M[f][:, f], G[f], W[f]
-    Ln = (Gf.T @ Mf @ Gf).toarray()
-    MG = (Mf @ Gf).toarray()
-    Wt = Wf - MG @ np.linalg.solve(Ln, Gf.T @ Wf)
-    # the penalty vanishes on gauged fields for any weight; this one matches K's scale
-    gamma = Kf.diagonal().mean() / Mf.diagonal().mean()
-    A = np.linalg.solve(Kf.toarray() + gamma * MG @ np.linalg.solve(Ln, MG.T), Wt)
-    # gauged: G^T M A = 0 up to rounding of its own terms
-    assert np.all(np.abs(MG.T @ A) <= 1e-9 * (np.abs(MG.T) @ np.abs(A)))
-    return float(Wf @ A)
+    return float(inductance_matrix(km, nu, [dict(region=region, turns=turns, area=area,
+                                                 dir=direction)])[0, 0])
 
 
 @pytest.mark.parametrize("order", [1, 2])
