@@ -8,6 +8,10 @@ field:
 - ``probeB(A_sol, point)`` (:403-441): B in the tet that holds the point, or 0
   outside the mesh.
 - ``probeLine(A_sol, start, end, numPts, "B")`` (:505-532).
+- ``computeElementJ(A_sol, V_sol, omega, sigmaMap, sourceMap)`` (:128-275): J at
+  the element centroids, on the device.
+- ``mapElementsToNodes`` / ``combineRegionData`` (:277-401): per-region nodal
+  averages of element values, and their combination over regions.
 - ``computeMagneticEnergy(A_sol, MatLibData)`` (:544-611, :691-760): with
   linear materials only, 0.5 Re(A^H K A) with K assembled on the GPU. Otherwise
   it sums over elements the volume times the energy density at the element's B.
@@ -150,6 +154,108 @@ class PostProcessor:
             return np.zeros((3, ncols)).squeeze(), float("nan")
         B = self.computeElementB(a)[:, e, :].cpu().numpy()
         return (B[:, 0] if ncols == 1 else B), e + 1
+
+    def computeElementJ(self, A_sol, V_sol, omega, sigmaMap, sourceMap) -> torch.Tensor:
+        """(3, Ne) complex device tensor of the current density at the element
+        centroids (:128-275): J = Js + sigma (-1j omega A - grad V) where
+        |sigma| > 1e-12, Js elsewhere. A is the edge field at the centroid
+        (reference basis values at (1/4, 1/4, 1/4) mapped by J^-T), and grad V
+        the P1 gradient of V_sol (given on the Lagrange_P1 DOFs, numbered from
+        that space's offset), taken only where all four of the element's V
+        DOFs exist. The reference's lookup rules are kept. A numeric sigma
+        vector is indexed by the element when it has at least e entries, else
+        by the tag. A [3 x Ne] source is per element, a map per tag."""
+        from .fem_tables import lagrange_tet_p1, nedelec_tet_p1
+        dev = self.Mesh.P_dev.device
+        ne = self.Mesh.NumElements
+        tags = np.asarray(self.Mesh.RegionTags, dtype=np.int64)
+        # sigma per element, the reference's lookup
+        if isinstance(sigmaMap, dict):
+            sig = np.array([float(sigmaMap.get(int(t), 0.0)) for t in tags])
+        else:
+            vec = np.asarray(sigmaMap, dtype=np.float64).ravel()
+            e1 = np.arange(1, ne + 1)
+            sig = np.where(vec.size >= e1, vec[np.minimum(e1, vec.size) - 1],
+                           np.where(vec.size >= tags, vec[np.clip(tags, 1, vec.size) - 1], 0.0))
+        src = np.asarray(sourceMap) if not isinstance(sourceMap, dict) else None
+        if src is not None and src.ndim == 2 and src.shape[1] == ne:
+            js = src.T.astype(np.float64)
+        else:
+            js = np.zeros((ne, 3))
+            for t, v in (sourceMap.items() if isinstance(sourceMap, dict) else []):
+                js[tags == int(t)] = np.asarray(v, dtype=np.float64).ravel()[:3]
+        cond = np.abs(sig) > 1e-12
+        J = torch.from_numpy(js.astype(np.complex128)).to(dev)
+        if cond.any():
+            ctr = np.full((3, 1), 0.25)
+            val = torch.from_numpy(nedelec_tet_p1(ctr)[0][:, :, 0]).to(dev)      # (6, 3)
+            grad = torch.from_numpy(lagrange_tet_p1(ctr)[1][:, :, 0]).to(dev)    # (4, 3)
+            invJt = torch.from_numpy(np.transpose(self._Jinv, (0, 2, 1)).copy()).to(dev)
+            key = FunctionSpace("Nedelec", 1).toString()
+            dofs = self.DofHandler.DofMaps[key].to(torch.int64)
+            signs = self.Mesh.T2E_Sign.to(torch.float64)
+            a = A_sol if torch.is_tensor(A_sol) else torch.from_numpy(np.asarray(A_sol))
+            a = a.to(dev).to(torch.complex128)
+            coef = torch.where(dofs > 0, a[(dofs - 1).clamp(min=0)], 0) * signs
+            A_ref = coef @ val.to(torch.complex128)                               # (Ne, 3)
+            A_c = torch.einsum("eij,ej->ei", invJt.to(torch.complex128), A_ref)
+            gV = torch.zeros_like(A_c)
+            if V_sol is not None:
+                kv = FunctionSpace("Lagrange", 1).toString()
+                v = V_sol if torch.is_tensor(V_sol) else torch.from_numpy(np.asarray(V_sol))
+                v = v.to(dev).to(torch.complex128)
+                off = self.DofHandler.SpaceOffsets.get(kv, 0)
+                loc = self.DofHandler.DofMaps[kv].to(torch.int64) - off
+                ok = ((loc > 0) & (loc <= v.numel())).all(dim=1)
+                vc = v[(loc - 1).clamp(0, max(v.numel() - 1, 0))]
+                g_ref = vc @ grad.to(torch.complex128)
+                gV = torch.where(ok[:, None], torch.einsum("eij,ej->ei",
+                                                           invJt.to(torch.complex128), g_ref), 0)
+            sg = torch.from_numpy(sig).to(dev)
+            eddy = sg[:, None] * (-1j * omega * A_c - gV)
+            J = torch.where(torch.from_numpy(cond).to(dev)[:, None], J + eddy, J)
+        return J.T
+
+    def mapElementsToNodes(self, Val_elems) -> dict:
+        """{tag: (dims, Nn) nodal values}: per region, each node's value is the
+        mean of the element values of that region's elements around it, 0 at
+        nodes outside the region (:277-315)."""
+        v = Val_elems if torch.is_tensor(Val_elems) else torch.from_numpy(np.asarray(Val_elems))
+        if v.dim() == 1:
+            v = v[None, :]
+        ne, nn = self.Mesh.NumElements, self.Mesh.NumNodes
+        if v.shape[1] != ne:
+            raise ValueError(f"PostProcessor:DimensionMismatch Val_elems columns ({v.shape[1]}) "
+                             f"must match mesh elements ({ne}).")
+        dev = self.Mesh.P_dev.device
+        v = v.to(dev)
+        T0 = self.Mesh.T_dev.view(ne, 4).to(torch.int64) - 1
+        tags = self.Mesh.Tags_dev
+        out = {}
+        for tag in np.unique(np.asarray(self.Mesh.RegionTags)):
+            m = tags == int(tag)
+            Ts = T0[m].reshape(-1)
+            vs = v[:, m].repeat_interleave(4, dim=1)
+            sums = torch.zeros((v.shape[0], nn), dtype=v.dtype, device=dev).index_add_(1, Ts, vs)
+            cnt = torch.zeros(nn, dtype=torch.float64, device=dev).index_add_(
+                0, Ts, torch.ones(Ts.numel(), dtype=torch.float64, device=dev))
+            out[int(tag)] = torch.where(cnt > 0, sums / cnt.clamp(min=1).to(sums.dtype), 0)
+        return out
+
+    def combineRegionData(self, RegionDataMap: dict, targetRegions) -> torch.Tensor:
+        """Nodal values of the target regions in order, a later region's nonzero
+        nodes overwriting earlier ones (:360-401)."""
+        nn = self.Mesh.NumNodes
+        dev = self.Mesh.P_dev.device
+        present = [RegionDataMap[int(t)] for t in np.atleast_1d(targetRegions)
+                   if int(t) in RegionDataMap]
+        dims = present[0].shape[0] if present else 1
+        dt = present[0].dtype if present else torch.float64
+        out = torch.zeros((dims, nn), dtype=dt, device=dev)
+        for vals in present:
+            mask = (vals != 0)[0] if dims == 1 else vals.abs().sum(dim=0) > 0
+            out[:, mask] = vals[:, mask]
+        return out
 
     def computeMagneticEnergy(self, A_sol, MatLibData) -> float:
         """Magnetic energy [J] of the field A_sol (real or complex)."""
