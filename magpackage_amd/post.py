@@ -169,14 +169,7 @@ class PostProcessor:
         dev = self.Mesh.P_dev.device
         ne = self.Mesh.NumElements
         tags = np.asarray(self.Mesh.RegionTags, dtype=np.int64)
-        # sigma per element, the reference's lookup
-        if isinstance(sigmaMap, dict):
-            sig = np.array([float(sigmaMap.get(int(t), 0.0)) for t in tags])
-        else:
-            vec = np.asarray(sigmaMap, dtype=np.float64).ravel()
-            e1 = np.arange(1, ne + 1)
-            sig = np.where(vec.size >= e1, vec[np.minimum(e1, vec.size) - 1],
-                           np.where(vec.size >= tags, vec[np.clip(tags, 1, vec.size) - 1], 0.0))
+        sig = element_sigma(sigmaMap, tags)
         src = np.asarray(sourceMap) if not isinstance(sourceMap, dict) else None
         if src is not None and src.ndim == 2 and src.shape[1] == ne:
             js = src.T.astype(np.float64)
@@ -314,6 +307,23 @@ class PostProcessor:
         ok = idx >= 0
         out[ok] = B[:, idx[ok]].T
         return out
+
+
+def element_sigma(sigmaMap, tags) -> np.ndarray:
+    """Per-element sigma as computeElementJ looks it up (PostProcessor.m:185-194):
+    a map by tag (0 when absent); a numeric vector by the element index e
+    (1-based) when it has at least e entries, else by the tag when it has at
+    least tag entries, else 0."""
+    tags = np.asarray(tags, dtype=np.int64)
+    ne = tags.size
+    if isinstance(sigmaMap, dict):
+        return np.array([float(sigmaMap.get(int(t), 0.0)) for t in tags])
+    vec = np.asarray(sigmaMap, dtype=np.float64).ravel()
+    if vec.size == 0:
+        return np.zeros(ne)
+    e1 = np.arange(1, ne + 1)
+    return np.where(vec.size >= e1, vec[np.minimum(e1, vec.size) - 1],
+                    np.where(vec.size >= tags, vec[np.clip(tags, 1, vec.size) - 1], 0.0))
 
 
 def _element_volumes(mesh) -> np.ndarray:

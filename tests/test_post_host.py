@@ -1,0 +1,38 @@
+"""Host-side logic of the post-processing and static solvers, on the CPU:
+element volumes (LossCalculator.computeElementVolumes), the reference's sigma
+lookup in computeElementJ, and the fixed-DOF index parsing of
+NonlinearSolver."""
+from types import SimpleNamespace
+
+import numpy as np
+
+from magpackage_amd.meshgen import kuhn_box
+from magpackage_amd.nonlinear import _index_list
+from magpackage_amd.post import _element_volumes, element_sigma
+
+
+def test_element_volumes_tile_the_box():
+    km = kuhn_box(4, 3, 5, lengths=(0.4, 0.3, 1.1), jitter=0.2, seed=4)
+    v = _element_volumes(SimpleNamespace(P=km.P, T=km.T))
+    assert np.all(v > 0)
+    assert abs(v.sum() - 0.4 * 0.3 * 1.1) <= 1e-14
+    one = _element_volumes(SimpleNamespace(P=np.array([[0, 0, 0], [2, 0, 0], [0, 3, 0],
+                                                      [0, 0, 4.0]]),
+                                           T=np.array([[1, 3, 2, 4]])))
+    assert abs(one[0] - 4.0) <= 1e-15  # 2*3*4/6, either orientation
+
+
+def test_sigma_lookup_follows_the_reference():
+    tags = np.array([2, 1, 2, 2, 1])
+    assert np.array_equal(element_sigma({2: 5.0}, tags), [5.0, 0, 5.0, 5.0, 0])
+    # per element when the vector is long enough
+    assert np.array_equal(element_sigma(np.arange(5.0) + 1, tags), [1, 2, 3, 4, 5])
+    # a per-tag vector: indexed by the element while e <= its length, then by tag
+    assert np.array_equal(element_sigma([10.0, 20.0], tags), [10, 20, 20, 20, 10])
+    assert np.array_equal(element_sigma([], tags), np.zeros(5))
+
+
+def test_fixed_dof_parsing():
+    assert np.array_equal(_index_list(np.array([True, False, True]), 10), [0, 2])
+    assert np.array_equal(_index_list([3, 1, 12, 0], 10), [2, 0])
+    assert _index_list(None, 10).size == 0 and _index_list([], 10).size == 0
