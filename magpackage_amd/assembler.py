@@ -7,7 +7,7 @@ argument meaning and the same results (CSC matrices equal to what
 - ``Mesh``          src/Mesh.m  (P, T, RegionTags; ``generate_edges`` :114-138)
 - ``FunctionSpace`` src/FunctionSpace.m
 - ``FemConfig``     src/FemConfig.m:8-17
-- ``DofHandler``    src/DofHandler.m:23-130 (Nedelec spaces)
+- ``DofHandler``    src/DofHandler.m:23-130 (Nedelec and Lagrange P1 spaces)
 - ``Assembler``     src/Assembler.m:26-76, 182-268
 
 Device arrays are ``torch`` tensors on ``cuda:0`` (torch is used for device
@@ -79,12 +79,16 @@ class Mesh:
     @staticmethod
     def load(filename: str, unit_tag: str = "m") -> "Mesh":
         """Mesh.load (src/Mesh.m:29-63): a COMSOL .mphtxt mesh, coordinates
-        scaled to metres.  Gmsh .msh input (read_msh) is not supported."""
-        from .mesh_io import read_mphtxt
+        scaled to metres, or a Gmsh .msh mesh, read unscaled (a unit other
+        than metres is warned about, as the reference does)."""
+        from .mesh_io import read_mphtxt, read_msh
         if str(filename).lower().endswith(".mphtxt"):
             raw = read_mphtxt(str(filename), unit_tag)
         elif str(filename).lower().endswith(".msh"):
-            raise NotImplementedError("Gmsh .msh meshes are not supported; use .mphtxt")
+            raw = read_msh(str(filename))
+            if str(unit_tag).lower() != "m":
+                import warnings
+                warnings.warn("read_msh does not scale coordinates; check the unit")
         else:
             raise ValueError(f"unsupported mesh format: {filename}")
         return Mesh(raw.P, raw.T, raw.RegionTags)

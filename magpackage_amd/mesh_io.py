@@ -1,4 +1,5 @@
-"""COMSOL text mesh reader (src/mesh_io/read_mphtxt.m:1-158).
+"""COMSOL text mesh reader (src/mesh_io/read_mphtxt.m:1-158) and the Gmsh
+reader (``read_msh``, src/mesh_io/read_msh.m:1-128, described at the function).
 
 ``read_mphtxt(filename, unit_tag="m")`` returns a ``MeshRaw``:
 - P: (Nn, 3) float64, the vertex coordinates scaled to metres;
@@ -170,4 +171,139 @@ def read_mphtxt(filename: str, unit_tag: str = "m") -> MeshRaw:
     tags = (np.concatenate(tet_tags) if tet_tags else np.zeros(0)).astype(np.int32)
     F = (np.concatenate(tris) if tris else np.zeros((0, 3), dtype=np.int64)).astype(np.int32)
     ftags = (np.concatenate(tri_tags) if tri_tags else np.zeros(0)).astype(np.int32)
+    return MeshRaw(P=P, T=T, RegionTags=tags, Faces=F, FaceTags=ftags)
+
+
+class _Stream:
+    """MATLAB's fgetl / fscanf over one text buffer: ``line()`` returns the rest
+    of the current line (after what fscanf consumed), ``numbers(k)`` reads up to
+    k whitespace-separated numbers across lines and stops at the first token
+    that is not a number, leaving the position right after the last number."""
+
+    def __init__(self, text: str):
+        self.s = text
+        self.i = 0
+
+    def eof(self) -> bool:
+        return self.i >= len(self.s)
+
+    def line(self) -> str:
+        j = self.s.find("\n", self.i)
+        if j < 0:
+            j = len(self.s)
+        out = self.s[self.i:j]
+        self.i = j + 1
+        return out
+
+    def numbers(self, k: int) -> list:
+        out = []
+        n = len(self.s)
+        while len(out) < k:
+            while self.i < n and self.s[self.i].isspace():
+                self.i += 1
+            if self.i >= n:
+                break
+            j = self.i
+            while j < n and not self.s[j].isspace():
+                j += 1
+            try:
+                out.append(float(self.s[self.i:j]))
+            except ValueError:
+                break
+            self.i = j
+        return out
+
+
+def _sscanf_floats(line: str) -> list:
+    """sscanf(line, '%f'): the leading run of numbers."""
+    out = []
+    for tok in line.split():
+        try:
+            out.append(float(tok))
+        except ValueError:
+            break
+    return out
+
+
+def read_msh(filename: str) -> MeshRaw:
+    """Gmsh mesh reader (src/mesh_io/read_msh.m:1-128), the reference's reading
+    of the format:
+    - ``$Nodes``: a line (blocks, nodes, ...), then per block a header line
+      whose 4th number is the block's node count, followed by that many
+      (tag, x, y, z) quadruples. Coordinates are kept in file order;
+    - ``$Elements``: a line (blocks, ...), then per block a header line
+      (dim, entity tag, element type, count). Type 2 (3-node triangle, rows
+      of id + 3 nodes) goes to Faces/FaceTags and type 4 (tetrahedron, id + 4
+      nodes) to T/RegionTags, tagged by the block's entity tag. Types 1 and
+      15 are skipped by their sizes; other types are warned about and not
+      skipped, as in the reference.
+    Node ids are used as given (1-based indices into the coordinates)."""
+    if not os.path.isfile(filename):
+        raise FileNotFoundError(f"File not found: {filename}")
+    with open(filename, "r") as f:
+        st = _Stream(f.read())
+
+    def header():
+        while not st.eof():
+            line = st.line().strip()
+            if not line:
+                continue
+            v = _sscanf_floats(line)
+            if v:
+                return v
+        return []
+
+    P = np.zeros((0, 3))
+    T_list, tag_list, F_list, ftag_list = [], [], [], []
+    while not st.eof():
+        line = st.line().strip()
+        if not line:
+            continue
+        if line == "$Nodes":
+            info = header()
+            if not info:
+                continue
+            nblocks, nnodes = int(info[0]), int(info[1])
+            P = np.zeros((nnodes, 3))
+            cnt = 0
+            for _ in range(nblocks):
+                b = header()
+                k = int(b[3])
+                data = st.numbers(4 * k)
+                if len(data) != 4 * k:
+                    raise ValueError(f"node block: expected {4 * k} numbers, read {len(data)}")
+                d = np.asarray(data).reshape(k, 4)
+                P[cnt:cnt + k] = d[:, 1:4]
+                cnt += k
+        elif line == "$Elements":
+            info = header()
+            if not info:
+                continue
+            for _ in range(int(info[0])):
+                b = header()
+                ent, etype, k = int(b[1]), int(b[2]), int(b[3])
+                if etype == 2:
+                    data = st.numbers(4 * k)
+                    if len(data) != 4 * k:
+                        raise ValueError("triangle block: short read")
+                    F_list.append(np.asarray(data).reshape(k, 4)[:, 1:4])
+                    ftag_list.append(np.full(k, ent))
+                elif etype == 4:
+                    data = st.numbers(5 * k)
+                    if len(data) != 5 * k:
+                        raise ValueError("tetrahedron block: short read")
+                    T_list.append(np.asarray(data).reshape(k, 5)[:, 1:5])
+                    tag_list.append(np.full(k, ent))
+                elif etype in (1, 15):
+                    st.numbers((3 if etype == 1 else 2) * k)
+                else:
+                    warnings.warn(f"unknown element type {etype}; not skipped")
+    T = (np.concatenate(T_list).astype(np.int32) if T_list
+         else np.zeros((0, 4), dtype=np.int32))
+    tags = (np.concatenate(tag_list).astype(np.int32) if tag_list
+            else np.zeros(0, dtype=np.int32))
+    F = (np.concatenate(F_list).astype(np.int32) if F_list
+         else np.zeros((0, 3), dtype=np.int32))
+    ftags = (np.concatenate(ftag_list).astype(np.int32) if ftag_list
+             else np.zeros(0, dtype=np.int32))
     return MeshRaw(P=P, T=T, RegionTags=tags, Faces=F, FaceTags=ftags)
