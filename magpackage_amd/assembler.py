@@ -593,6 +593,45 @@ class Assembler:
         k = int(nnz.value)
         return CscMatrix((n, n), colptr, rowind[:k], vals[:k])
 
+    def assembleCoupling(self, spaceRow, spaceCol, coeffMap) -> CscMatrix:
+        """C = assembleCoupling(spaceRow, spaceCol, coeffMap) (src/Assembler.m:99-119):
+        assemble_coupling_kernel_mex (Nedelec rows, Lagrange columns) at the
+        config's quadrature order (at least 2), then sparse(I(valid), J(valid),
+        V(valid), N, N) as a device CSC matrix."""
+        from .fem_tables import get_quadrature_data, lagrange_tet_p1, matlab_flat
+        if spaceRow.Type.lower() != "nedelec" or spaceCol.Type.lower() != "lagrange":
+            raise ValueError("Unsupported space combination")
+        mesh = self.Mesh
+        dh = self.DofHandler
+        n = dh.NumGlobalDofs
+        ne, nn = mesh.NumElements, mesh.NumNodes
+        dev = _dev()
+        order = max(self.Config.DefaultQuadratureOrder, 2)
+        q_pts, _ = get_quadrature_data("tet", order)
+        grad = np.ascontiguousarray(matlab_flat(lagrange_tet_p1(q_pts)[1]))
+        q = L.standard_quadrature(order)
+        coeff = torch.from_numpy(expand_material_map(coeffMap, mesh.RegionTags)).to(dev)
+        Td = mesh.T_dev.to(torch.float64)
+        De = dh.DofMaps[spaceRow.toString()].reshape(-1).to(torch.float64)
+        Sd = mesh.T2E_Sign.reshape(-1).to(torch.float64)
+        Dn = dh.DofMaps[spaceCol.toString()].reshape(-1).to(torch.float64)
+        nt = 24 * ne
+        Io = torch.empty(max(nt, 1), dtype=torch.float64, device=dev)
+        Jo, Vo = torch.empty_like(Io), torch.empty_like(Io)
+        L.check(L.lib.mpk_coupling_coo(
+            L.ptr(mesh.P_dev), nn, L.ptr(Td), ne, L.ptr(De), L.ptr(Sd), L.ptr(Dn), L.ptr(coeff),
+            C.byref(q), grad.ctypes.data_as(C.c_void_p), L.ptr(Io), L.ptr(Jo), L.ptr(Vo),
+            L.stream_ptr()))
+        colptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        rowind = torch.empty(max(nt, 1), dtype=torch.int32, device=dev)
+        vals = torch.empty(max(nt, 1), dtype=torch.float64, device=dev)
+        nnz = C.c_int64(0)
+        L.check(L.lib.mpk_sparse_coo_to_csc(L.ptr(Io), L.ptr(Jo), L.ptr(Vo), nt, n, n, 1,
+                                            L.ptr(colptr), L.ptr(rowind), L.ptr(vals), nt,
+                                            C.byref(nnz), L.stream_ptr()))
+        k = int(nnz.value)
+        return CscMatrix((n, n), colptr, rowind[:k], vals[:k])
+
     def assembleSource(self, space, sourceMap, out: torch.Tensor | None = None):
         """F = assembleSource(space, sourceMap) (src/Assembler.m:78-97): the
         dense [NumGlobalDofs] source vector on the device."""
