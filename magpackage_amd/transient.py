@@ -177,6 +177,8 @@ class TransientSolver:
         self.LinearEtaFloor = 0.5
         self.LinearMaxIter = 20000
         self.LinearCheckEvery = 10
+        # solveAV: relative CG tolerance of the A-V Newton solves
+        self.LinearRelTolAV = 1e-12
         # "aux": Jacobi + gradient-space correction (mpk_pcg_aux_solve);
         # "jacobi": plain Jacobi-PCG
         self.LinearPreconditioner = "aux"
@@ -425,6 +427,199 @@ class TransientSolver:
                           f" | PCG per solve {info.pcg_per_solve}", file=sys.stderr, flush=True)
         return results, infos
 
+    def solveAV(self, space_A, space_V, matLib, sigmaMap, sourceFunc, timeSteps, fixedDofs_A,
+                  fixedDofs_V=None, init_A=None, init_V=None):
+        """TransientSolver.solve with the scalar-potential space (src/solver/
+        TransientSolver.m:35-267, evaluateSystemState :287-311): unknowns
+        [A; V], V on the nodes of the conducting regions.
+
+            R = R_nu + M_sigma (x - x_prev)/dt - F + C V + [C^T (A - A_prev)/dt + L V]
+                + reg M_reg x
+            J = J_nu + M_sigma/dt + C + C^T/dt + L + reg M_reg
+
+        with C = assembleCoupling(A, V, sigma) and L = assembleScalarLaplacian(V,
+        sigma), reg from the mean positive |diag J| at the first iteration of the
+        first step, V grounded at the given DOFs or the first V DOF. With
+        V = W/dt the Newton matrix is J_nu + reg M_reg + (1/dt)[[M_sigma, C],
+        [C^T, L]] in (A, W): symmetric positive definite once grounded, so the
+        engine's auxiliary-space CG solves it (to LinearRelTolAV) on a merged CSC
+        pattern (the plan's edge block, C, C^T, L). Linear materials skip the line
+        search and stop below 1e-9 after a step, as the reference does.
+        Returns (solutions [N device tensors], infos)."""
+        import scipy.sparse as sp
+        asm = self.Assembler
+        dh = asm.DofHandler
+        mesh = asm.Mesh
+        dh.distributeDofs(space_A)
+        nA = dh.SpaceLocalSizes[space_A.toString()]
+        from .frequency import FrequencySolver
+        tags_c = FrequencySolver.getConductingRegions(sigmaMap)
+        if not tags_c or space_V is None:
+            raise ValueError("solveAV needs a scalar-potential space and a conducting region; "
+                             "use solve() for the edge space alone")
+        kv = space_V.toString()
+        if kv not in dh.DofMaps:
+            _, offset_V = dh.distributeDofs(space_V, tags_c)
+        else:
+            offset_V = dh.SpaceOffsets[kv]
+        N = dh.NumGlobalDofs
+        from .nonlinear import _index_list
+        fv = _index_list(fixedDofs_V, N)
+        fixed = np.zeros(N, dtype=bool)
+        fixed[_index_list(fixedDofs_A, N)] = True
+        if fv.size:
+            fixed[fv] = True
+        else:
+            fixed[offset_V] = True
+        self._setup(space_A, matLib, sigmaMap, fixed)
+        if self.n != N:
+            raise RuntimeError(f"plan has {self.n} DOFs, NumGlobalDofs is {N}")
+        dev = self.dev
+        Cm = asm.assembleCoupling(space_A, space_V, sigmaMap)
+        Lm = asm.assembleScalarLaplacian(space_V, sigmaMap)
+        pcp, pri = self.plan.pattern()
+        Cs = sp.csc_matrix((Cm.values.cpu().numpy(), Cm.rowind.cpu().numpy(),
+                            Cm.colptr.cpu().numpy()), shape=(N, N))
+        CT = Cs.T.tocsc()
+        CT.sort_indices()
+        Cs.sort_indices()
+        blocks = [(pcp.cpu().numpy().astype(np.int64), pri.cpu().numpy()),
+                  (Cs.indptr.astype(np.int64), Cs.indices), (CT.indptr.astype(np.int64), CT.indices),
+                  (Lm.colptr.cpu().numpy().astype(np.int64), Lm.rowind.cpu().numpy())]
+        mcp, mri, pos = _merged_pattern(N, blocks)
+        nnz_m = int(mcp[-1])
+        colptr = torch.from_numpy(mcp).to(dev)
+        rowind = torch.from_numpy(mri).to(dev)
+        pos_d = [torch.from_numpy(p).to(dev) for p in pos]
+        c_vals = torch.from_numpy(Cs.data).to(dev)
+        ct_vals = torch.from_numpy(CT.data).to(dev)
+        l_vals = Lm.values.to(torch.float64)
+        # symmetric C/C^T/L values for the residual's products
+        cl = torch.zeros(nnz_m, dtype=torch.float64, device=dev)
+        cl[pos_d[1]] = c_vals
+        cl[pos_d[2]] = ct_vals
+        cl[pos_d[3]] = l_vals
+        lvv_diag = Lm.to_scipy().diagonal()[nA:]
+        sysv = torch.zeros(nnz_m, dtype=torch.float64, device=dev)
+        cvals_m = torch.zeros(nnz_m, dtype=torch.float64, device=dev)
+        fixed_d = self.fixed
+        free = self.free
+        linear = all(getattr(m, "Type", "Linear") == "Linear" for m in dict(matLib).values())
+        x_prev = torch.zeros(N, dtype=torch.float64, device=dev)
+        if init_A is not None:
+            a0 = torch.as_tensor(init_A, dtype=torch.float64, device=dev).flatten()
+            x_prev[: min(nA, a0.numel())] = a0[:nA]
+        if init_V is not None:
+            v0 = torch.as_tensor(init_V, dtype=torch.float64, device=dev).flatten()
+            x_prev[offset_V: offset_V + v0.numel()] = v0
+        y1 = torch.empty(N, dtype=torch.float64, device=dev)
+        y2 = torch.empty_like(y1)
+        mx = torch.empty_like(y1)
+
+        def spmv_m(vals, x, y):
+            L.check(L.lib.mpk_spmv_sym(L.ptr(colptr), L.ptr(rowind), L.ptr(vals), N, L.ptr(x),
+                                       L.ptr(y), self.sp))
+
+        def residual(x, dt, F, calc_j):
+            self._jacobian(x, calc_j)
+            dx_ = x - x_prev
+            self._spmv(self.m_sigma, dx_, mx)
+            R = self.res + mx / dt - F
+            xv = torch.zeros_like(x)
+            xv[nA:] = x[nA:]
+            spmv_m(cl, xv, y1)                 # [C V; L V]
+            xa = torch.zeros_like(x)
+            xa[:nA] = dx_[:nA]
+            spmv_m(cl, xa, y2)                 # [0; C^T (A - A_prev)] (+ C 0)
+            y2[:nA] = 0.0
+            R = R + y1 + y2 / dt
+            if self.AutoRegularization and self.reg_abs > 0:
+                self._spmv(self.m_reg, x, mx)
+                R = R + self.reg_abs * mx
+            return R
+
+        results, infos = [], []
+        t_now = 0.0
+        aux_key = None
+        for step, dt in enumerate(np.asarray(timeSteps, dtype=np.float64).ravel(), start=1):
+            dt = float(dt)
+            t_now += dt
+            src = sourceFunc(t_now)
+            F = (src.to(torch.float64) if torch.is_tensor(src) and src.dim() == 1
+                 else asm.assembleSource(space_A, src))
+            x = x_prev.clone()
+            info = StepInfo(step, t_now, 0, 0, 0, math.inf, math.inf, False, 0.0)
+            initial = 1.0
+            for it in range(1, self.MaxIterations + 1):
+                R = residual(x, dt, F, True)
+                if self.reg_abs == 0 and self.AutoRegularization:
+                    self._combine(self.jsys, 1.0, self.jv, 1.0 / dt, self.m_sigma)
+                    d = np.concatenate([self.jsys[self.diag_pos].abs().cpu().numpy(),
+                                        np.abs(lvv_diag)])
+                    d = d[d > 0]
+                    avg = float(d.mean()) if d.size else 1.0
+                    self.reg_abs = (avg if math.isfinite(avg) and avg != 0 else 1.0) * self.RegScaleRel
+                    self._spmv(self.m_reg, x, mx)
+                    R = R + self.reg_abs * mx
+                Rb = R * free
+                norm_res = float(torch.linalg.vector_norm(Rb))
+                info.history.append(norm_res)
+                if it == 1:
+                    initial = norm_res if norm_res >= 1e-20 else 1.0
+                info.newton_iters = it
+                info.residual, info.rel_residual = norm_res, norm_res / initial
+                if norm_res < self.Tolerance or norm_res / initial < self.RelTolerance:
+                    info.converged = True
+                    break
+                # scaled symmetric Newton matrix on the merged pattern
+                self._combine(self.cvals, 1.0 / dt, self.m_sigma, self.reg_abs, self.m_reg)
+                self._combine(self.jsys, 1.0, self.jv, 1.0, self.cvals)
+                sysv.zero_()
+                sysv[pos_d[0]] = self.jsys[: pos_d[0].numel()]
+                sysv[pos_d[1]] = c_vals / dt
+                sysv[pos_d[2]] = ct_vals / dt
+                sysv[pos_d[3]] = l_vals / dt
+                key = (dt, self.reg_abs)
+                if key != aux_key:
+                    cvals_m.zero_()
+                    cvals_m[pos_d[0]] = self.cvals[: pos_d[0].numel()]
+                    aux, keep = build_aux_space(asm, space_A.toString(), fixed_d, cvals_m, colptr,
+                                                rowind, self.AuxSweeps, self.AuxOmega)
+                    aux_key = key
+                w = torch.zeros(N, dtype=torch.float64, device=dev)
+                itc, rel = C.c_int(), C.c_double()
+                L.check(L.lib.mpk_pcg_aux_solve(L.ptr(colptr), L.ptr(rowind), L.ptr(sysv), N,
+                                                L.ptr(fixed_d), C.byref(aux), L.ptr(-Rb), L.ptr(w),
+                                                L.ptr(keep["work"]), self.LinearRelTolAV,
+                                                self.LinearMaxIter, self.LinearCheckEvery,
+                                                C.byref(itc), C.byref(rel), self.sp))
+                info.pcg_iters += int(itc.value)
+                info.pcg_per_solve.append(int(itc.value))
+                dxv = w.clone()
+                dxv[nA:] = w[nA:] / dt       # V = W / dt
+                alpha = self.Relaxation
+                if self.UseLineSearch and not linear:
+                    for k in range(self.MaxLineSearchIters + 1):
+                        a_try = alpha * (0.5 ** k)
+                        xt = x + a_try * dxv
+                        rt = float(torch.linalg.vector_norm(residual(xt, dt, F, False) * free))
+                        info.line_search_evals += 1
+                        if rt < norm_res or (it == 1 and k < 3):
+                            alpha = a_try
+                            break
+                        if rt > norm_res * 10:
+                            alpha = alpha * 0.1
+                x = x + alpha * dxv
+                if norm_res > 1e20:
+                    raise RuntimeError("Divergence detected (Res > 1e20). Check BCs or Time Step.")
+                if linear and norm_res < 1e-9:
+                    info.converged = True
+                    break
+            infos.append(info)
+            results.append(x.clone())
+            x_prev = x
+        return results, infos
+
     def _newton(self, step, t, dt, x, b, rel_ref):
         info = StepInfo(step, t, 0, 0, 0, math.inf, math.inf, False, 0.0)
         initial_res = rel_ref
@@ -513,205 +708,3 @@ def _merged_pattern(n, blocks):
     slot_of[src] = np.arange(tot)
     pos = [slot_of[o:o + int(cp[-1])] for o, (cp, _) in zip(offs, blocks)]
     return Msum.indptr.astype(np.int32), Msum.indices.astype(np.int32), pos
-
-
-def _solve_av(self, space_A, space_V, matLib, sigmaMap, sourceFunc, timeSteps, fixedDofs_A,
-              fixedDofs_V=None, init_A=None, init_V=None):
-    """TransientSolver.solve with the scalar-potential space (src/solver/
-    TransientSolver.m:35-267, evaluateSystemState :287-311): unknowns
-    [A; V], V on the nodes of the conducting regions.
-
-        R = R_nu + M_sigma (x - x_prev)/dt - F + C V + [C^T (A - A_prev)/dt + L V]
-            + reg M_reg x
-        J = J_nu + M_sigma/dt + C + C^T/dt + L + reg M_reg
-
-    with C = assembleCoupling(A, V, sigma) and L = assembleScalarLaplacian(V,
-    sigma), reg from the mean positive |diag J| at the first iteration of the
-    first step, V grounded at the given DOFs or the first V DOF. With
-    V = W/dt the Newton matrix is J_nu + reg M_reg + (1/dt)[[M_sigma, C],
-    [C^T, L]] in (A, W): symmetric positive definite once grounded, so the
-    engine's auxiliary-space CG solves it (to LinearRelTolAV) on a merged CSC
-    pattern (the plan's edge block, C, C^T, L). Linear materials skip the line
-    search and stop below 1e-9 after a step, as the reference does.
-    Returns (solutions [N device tensors], infos)."""
-    import scipy.sparse as sp
-    asm = self.Assembler
-    dh = asm.DofHandler
-    mesh = asm.Mesh
-    dh.distributeDofs(space_A)
-    nA = dh.SpaceLocalSizes[space_A.toString()]
-    tags_c = FrequencySolver_getConductingRegions(sigmaMap)
-    if not tags_c or space_V is None:
-        raise ValueError("solveAV needs a scalar-potential space and a conducting region; "
-                         "use solve() for the edge space alone")
-    kv = space_V.toString()
-    if kv not in dh.DofMaps:
-        _, offset_V = dh.distributeDofs(space_V, tags_c)
-    else:
-        offset_V = dh.SpaceOffsets[kv]
-    N = dh.NumGlobalDofs
-    from .nonlinear import _index_list
-    fv = _index_list(fixedDofs_V, N)
-    fixed = np.zeros(N, dtype=bool)
-    fixed[_index_list(fixedDofs_A, N)] = True
-    if fv.size:
-        fixed[fv] = True
-    else:
-        fixed[offset_V] = True
-    self._setup(space_A, matLib, sigmaMap, fixed)
-    if self.n != N:
-        raise RuntimeError(f"plan has {self.n} DOFs, NumGlobalDofs is {N}")
-    dev = self.dev
-    Cm = asm.assembleCoupling(space_A, space_V, sigmaMap)
-    Lm = asm.assembleScalarLaplacian(space_V, sigmaMap)
-    pcp, pri = self.plan.pattern()
-    Cs = sp.csc_matrix((Cm.values.cpu().numpy(), Cm.rowind.cpu().numpy(),
-                        Cm.colptr.cpu().numpy()), shape=(N, N))
-    CT = Cs.T.tocsc()
-    CT.sort_indices()
-    Cs.sort_indices()
-    blocks = [(pcp.cpu().numpy().astype(np.int64), pri.cpu().numpy()),
-              (Cs.indptr.astype(np.int64), Cs.indices), (CT.indptr.astype(np.int64), CT.indices),
-              (Lm.colptr.cpu().numpy().astype(np.int64), Lm.rowind.cpu().numpy())]
-    mcp, mri, pos = _merged_pattern(N, blocks)
-    nnz_m = int(mcp[-1])
-    colptr = torch.from_numpy(mcp).to(dev)
-    rowind = torch.from_numpy(mri).to(dev)
-    pos_d = [torch.from_numpy(p).to(dev) for p in pos]
-    c_vals = torch.from_numpy(Cs.data).to(dev)
-    ct_vals = torch.from_numpy(CT.data).to(dev)
-    l_vals = Lm.values.to(torch.float64)
-    # symmetric C/C^T/L values for the residual's products
-    cl = torch.zeros(nnz_m, dtype=torch.float64, device=dev)
-    cl[pos_d[1]] = c_vals
-    cl[pos_d[2]] = ct_vals
-    cl[pos_d[3]] = l_vals
-    lvv_diag = Lm.to_scipy().diagonal()[nA:]
-    sysv = torch.zeros(nnz_m, dtype=torch.float64, device=dev)
-    cvals_m = torch.zeros(nnz_m, dtype=torch.float64, device=dev)
-    fixed_d = self.fixed
-    free = self.free
-    linear = all(getattr(m, "Type", "Linear") == "Linear" for m in dict(matLib).values())
-    x_prev = torch.zeros(N, dtype=torch.float64, device=dev)
-    if init_A is not None:
-        a0 = torch.as_tensor(init_A, dtype=torch.float64, device=dev).flatten()
-        x_prev[: min(nA, a0.numel())] = a0[:nA]
-    if init_V is not None:
-        v0 = torch.as_tensor(init_V, dtype=torch.float64, device=dev).flatten()
-        x_prev[offset_V: offset_V + v0.numel()] = v0
-    y1 = torch.empty(N, dtype=torch.float64, device=dev)
-    y2 = torch.empty_like(y1)
-    mx = torch.empty_like(y1)
-
-    def spmv_m(vals, x, y):
-        L.check(L.lib.mpk_spmv_sym(L.ptr(colptr), L.ptr(rowind), L.ptr(vals), N, L.ptr(x),
-                                   L.ptr(y), self.sp))
-
-    def residual(x, dt, F, calc_j):
-        self._jacobian(x, calc_j)
-        dx_ = x - x_prev
-        self._spmv(self.m_sigma, dx_, mx)
-        R = self.res + mx / dt - F
-        xv = torch.zeros_like(x)
-        xv[nA:] = x[nA:]
-        spmv_m(cl, xv, y1)                 # [C V; L V]
-        xa = torch.zeros_like(x)
-        xa[:nA] = dx_[:nA]
-        spmv_m(cl, xa, y2)                 # [0; C^T (A - A_prev)] (+ C 0)
-        y2[:nA] = 0.0
-        R = R + y1 + y2 / dt
-        if self.AutoRegularization and self.reg_abs > 0:
-            self._spmv(self.m_reg, x, mx)
-            R = R + self.reg_abs * mx
-        return R
-
-    results, infos = [], []
-    t_now = 0.0
-    aux_key = None
-    for step, dt in enumerate(np.asarray(timeSteps, dtype=np.float64).ravel(), start=1):
-        dt = float(dt)
-        t_now += dt
-        src = sourceFunc(t_now)
-        F = (src.to(torch.float64) if torch.is_tensor(src) and src.dim() == 1
-             else asm.assembleSource(space_A, src))
-        x = x_prev.clone()
-        info = StepInfo(step, t_now, 0, 0, 0, math.inf, math.inf, False, 0.0)
-        initial = 1.0
-        for it in range(1, self.MaxIterations + 1):
-            R = residual(x, dt, F, True)
-            if self.reg_abs == 0 and self.AutoRegularization:
-                self._combine(self.jsys, 1.0, self.jv, 1.0 / dt, self.m_sigma)
-                d = np.concatenate([self.jsys[self.diag_pos].abs().cpu().numpy(),
-                                    np.abs(lvv_diag)])
-                d = d[d > 0]
-                avg = float(d.mean()) if d.size else 1.0
-                self.reg_abs = (avg if math.isfinite(avg) and avg != 0 else 1.0) * self.RegScaleRel
-                self._spmv(self.m_reg, x, mx)
-                R = R + self.reg_abs * mx
-            Rb = R * free
-            norm_res = float(torch.linalg.vector_norm(Rb))
-            info.history.append(norm_res)
-            if it == 1:
-                initial = norm_res if norm_res >= 1e-20 else 1.0
-            info.newton_iters = it
-            info.residual, info.rel_residual = norm_res, norm_res / initial
-            if norm_res < self.Tolerance or norm_res / initial < self.RelTolerance:
-                info.converged = True
-                break
-            # scaled symmetric Newton matrix on the merged pattern
-            self._combine(self.cvals, 1.0 / dt, self.m_sigma, self.reg_abs, self.m_reg)
-            self._combine(self.jsys, 1.0, self.jv, 1.0, self.cvals)
-            sysv.zero_()
-            sysv[pos_d[0]] = self.jsys[: pos_d[0].numel()]
-            sysv[pos_d[1]] = c_vals / dt
-            sysv[pos_d[2]] = ct_vals / dt
-            sysv[pos_d[3]] = l_vals / dt
-            key = (dt, self.reg_abs)
-            if key != aux_key:
-                cvals_m.zero_()
-                cvals_m[pos_d[0]] = self.cvals[: pos_d[0].numel()]
-                aux, keep = build_aux_space(asm, space_A.toString(), fixed_d, cvals_m, colptr,
-                                            rowind, self.AuxSweeps, self.AuxOmega)
-                aux_key = key
-            w = torch.zeros(N, dtype=torch.float64, device=dev)
-            itc, rel = C.c_int(), C.c_double()
-            L.check(L.lib.mpk_pcg_aux_solve(L.ptr(colptr), L.ptr(rowind), L.ptr(sysv), N,
-                                            L.ptr(fixed_d), C.byref(aux), L.ptr(-Rb), L.ptr(w),
-                                            L.ptr(keep["work"]), self.LinearRelTolAV,
-                                            self.LinearMaxIter, self.LinearCheckEvery,
-                                            C.byref(itc), C.byref(rel), self.sp))
-            info.pcg_iters += int(itc.value)
-            info.pcg_per_solve.append(int(itc.value))
-            dxv = w.clone()
-            dxv[nA:] = w[nA:] / dt       # V = W / dt
-            alpha = self.Relaxation
-            if self.UseLineSearch and not linear:
-                for k in range(self.MaxLineSearchIters + 1):
-                    a_try = alpha * (0.5 ** k)
-                    xt = x + a_try * dxv
-                    rt = float(torch.linalg.vector_norm(residual(xt, dt, F, False) * free))
-                    info.line_search_evals += 1
-                    if rt < norm_res or (it == 1 and k < 3):
-                        alpha = a_try
-                        break
-                    if rt > norm_res * 10:
-                        alpha = alpha * 0.1
-            x = x + alpha * dxv
-            if norm_res > 1e20:
-                raise RuntimeError("Divergence detected (Res > 1e20). Check BCs or Time Step.")
-            if linear and norm_res < 1e-9:
-                info.converged = True
-                break
-        infos.append(info)
-        results.append(x.clone())
-        x_prev = x
-    return results, infos
-
-
-def FrequencySolver_getConductingRegions(sigmaMap):
-    from .frequency import FrequencySolver
-    return FrequencySolver.getConductingRegions(sigmaMap)
-
-
-TransientSolver.LinearRelTolAV = 1e-12
-TransientSolver.solveAV = _solve_av
