@@ -58,3 +58,66 @@ def test_nonlinear_eddy_steps_converge():
     sols, iters = transient_cpu(m, lib, sigma, src, [1e-3] * 3, fixed)
     assert max(iters) >= 3
     assert all(np.isfinite(s).all() for s in sols)
+
+
+def test_av_restatement_is_gauge_equivalent_to_pure_a():
+    """The A-V restatement (tests/transient_ref.transient_av_cpu) against the
+    pure-A one. The gradients of the P1 potential lie in the edge space, so
+    A* = A + G (dt sum V) carries the same B when the conductor stays off the
+    Dirichlet boundary and the load is discretely divergence-free. The two
+    then agree up to the regularisation: B to ~eps, both at eps = 1e-8 here."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from oracle import oracle as O
+    from magpackage_amd.material import create_linear, marrocco
+    from tests.circuit_ref import boundary_nodes
+    from tests.helpers import core_box_mesh, mass_tables, stiffness_tables
+    from tests.nonlinear_ref import outer_fixed_mask
+    from tests.transient_ref import transient_av_cpu, transient_cpu
+    km = core_box_mesh(6, seed=2)
+    assert not np.isin(np.unique(km.T[km.RegionTags == 2] - 1), boundary_nodes(km.T)).any()
+    lib = {1: create_linear(1.0), 2: marrocco()}
+    sig = np.where(km.RegionTags == 2, 2e6, 0.0)
+    fixed = outer_fixed_mask(km)
+    mi = O.mesh_inputs(km)
+    n = mi["ndof"]
+    P, T, D, S = mi["P"], mi["T"], mi["Dofs"], mi["Signs"]
+    qm, val, _ = mass_tables(2)
+    js = np.zeros((km.T.shape[0], 3))
+    js[km.RegionTags == 1] = [1.0, 0.3, 0.0]
+    I, V = O.source(P, T, D, S, js, qm, val)
+    W = np.zeros(n)
+    np.add.at(W, I.astype(np.int64) - 1, V)
+    M = O.sparse_scipy(*O.mass(P, T, D, S, None, qm, val), n, n, drop_zeros=False).tocsr()
+    edges = mi["edges"]
+    emap = np.zeros(edges.shape[0], dtype=np.int64)
+    emap[mi["t2e"].ravel() - 1] = mi["cell"].ravel()
+    fn = np.setdiff1d(np.arange(km.P.shape[0]), boundary_nodes(km.T))
+    fid = np.full(km.P.shape[0], -1)
+    fid[fn] = np.arange(fn.size)
+    rows = emap - 1
+    a, b = fid[edges[:, 0] - 1], fid[edges[:, 1] - 1]
+    ma, mb = (a >= 0) & (rows >= 0), (b >= 0) & (rows >= 0)
+    G = sp.csr_matrix((np.r_[-np.ones(ma.sum()), np.ones(mb.sum())],
+                       (np.r_[rows[ma], rows[mb]], np.r_[a[ma], b[mb]])), shape=(n, fn.size))
+    w = np.where(fixed, 0.0, W)
+    Wt = w - M @ (G @ spla.spsolve((G.T @ M @ G).tocsc(), G.T @ w))
+    Wt[fixed] = 0.0
+    assert np.abs(G.T @ Wt).max() <= 1e-12 * np.abs(G.T @ w).max()
+
+    def load(t):
+        return Wt * (3e6 * np.sin(2 * np.pi * 50 * t))
+
+    steps = [1e-3] * 3
+    av, its_av, nA = transient_av_cpu(km, lib, sig, None, steps, fixed, tol=0, rel_tol=1e-11,
+                                      reg_rel=1e-8, load=load, max_iter=12)
+    pa, its_a = transient_cpu(km, lib, sig, None, steps, fixed, tol=0, rel_tol=1e-11,
+                              reg_rel=1e-8, load=load, max_iter=12)
+    curl = stiffness_tables()[1]
+    for k in range(len(steps)):
+        B1 = O.element_b(P, T, D, S, av[k][:nA], curl)
+        B2 = O.element_b(P, T, D, S, pa[k], curl)
+        err = np.abs(B1 - B2).max() / np.abs(B2).max()
+        print(f"step {k + 1}: Newton {its_av[k]} / {its_a[k]}, B rel diff {err:.1e}")
+        assert err <= 1e-7
+        assert np.abs(av[k][nA:]).max() > 0  # the potential carries the gradient part

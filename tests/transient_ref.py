@@ -30,8 +30,9 @@ def apply_dirichlet(K, F, fixed):
 
 
 def transient_cpu(mesh, lib, sigma_e, source, time_steps, fixed, tol=1e-5, rel_tol=1e-6,
-                  max_iter=50, reg_rel=1e-5, max_ls=10):
-    """source(t) -> (Ne, 3) current density.  Returns (solutions, newton iteration counts)."""
+                  max_iter=50, reg_rel=1e-5, max_ls=10, load=None):
+    """source(t) -> (Ne, 3) current density, or load(t) -> the assembled
+    source vector [n].  Returns (solutions, newton iteration counts)."""
     mi = O.mesh_inputs(mesh)
     n = mi["ndof"]
     P, T, D, S = mi["P"], mi["T"], mi["Dofs"], mi["Signs"]
@@ -43,6 +44,8 @@ def transient_cpu(mesh, lib, sigma_e, source, time_steps, fixed, tol=1e-5, rel_t
     Mreg = O.sparse_scipy(*O.mass(P, T, D, S, None, qm, val), n, n, drop_zeros=False)
 
     def assemble_source(t):
+        if load is not None:
+            return np.asarray(load(t), dtype=np.float64)
         I, V = O.source(P, T, D, S, source(t), qm, val)
         F = np.zeros(n)
         np.add.at(F, I.astype(np.int64) - 1, V)
@@ -103,7 +106,7 @@ def transient_cpu(mesh, lib, sigma_e, source, time_steps, fixed, tol=1e-5, rel_t
 
 def transient_av_cpu(mesh, lib, sigma_e, source, time_steps, fixed_A, v_fixed_nodes=None,
                      tol=1e-5, rel_tol=1e-6, max_iter=50, reg_rel=1e-5, max_ls=10,
-                     line_search=True):
+                     line_search=True, load=None):
     """TransientSolver.solve with the scalar-potential space
     (src/solver/TransientSolver.m:35-267, evaluateSystemState :287-311).
     Unknowns [A (1..nA); V (nodes of the conducting elements, increasing
@@ -153,8 +156,11 @@ def transient_av_cpu(mesh, lib, sigma_e, source, time_steps, fixed_A, v_fixed_no
         fixed[d[d > 0] - 1] = True
 
     def assemble_source(t):
-        I, V = O.source(P, T, D, S, source(t), qm, val)
         F = np.zeros(N)
+        if load is not None:
+            F[:nA] = np.asarray(load(t), dtype=np.float64)[:nA]
+            return F
+        I, V = O.source(P, T, D, S, source(t), qm, val)
         np.add.at(F, I.astype(np.int64) - 1, V)
         return F
 
