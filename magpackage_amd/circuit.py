@@ -74,6 +74,23 @@ class Winding:
         self.DirectionField = np.asarray(field, dtype=np.float64)
 
 
+class CurrentSource:
+    """CurrentSource (src/CurrentSource.m:1-20): a 'Domain' current density
+    J_Vector [Jx, Jy, Jz] on one region.  ``as_source_map`` gives the {tag: J}
+    form that Assembler.assembleSource accepts."""
+
+    def __init__(self, regionID, J_vec):
+        J = np.asarray(J_vec, dtype=np.float64).ravel()
+        if J.size != 3:
+            raise ValueError("J_Vector must be a 3-element vector.")
+        self.Type = "Domain"
+        self.RegionID = regionID
+        self.J_Vector = J
+
+    def as_source_map(self) -> dict:
+        return {int(self.RegionID): self.J_Vector.copy()}
+
+
 class Circuit:
     """One entry of the reference's circuit struct array: R, L and
     V_source_func(t)."""

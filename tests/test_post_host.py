@@ -36,3 +36,13 @@ def test_fixed_dof_parsing():
     assert np.array_equal(_index_list(np.array([True, False, True]), 10), [0, 2])
     assert np.array_equal(_index_list([3, 1, 12, 0], 10), [2, 0])
     assert _index_list(None, 10).size == 0 and _index_list([], 10).size == 0
+
+
+def test_current_source():
+    import pytest
+    from magpackage_amd.circuit import CurrentSource
+    cs = CurrentSource(5, [0, 0, 1e6])
+    assert cs.Type == "Domain" and cs.RegionID == 5
+    assert np.array_equal(cs.as_source_map()[5], [0.0, 0.0, 1e6])
+    with pytest.raises(ValueError):
+        CurrentSource(1, [1.0, 2.0])
