@@ -348,7 +348,6 @@ class LossCalculator:
 
     def _quad_form(self, vals, colptr, rowind, a) -> float:
         """Re(a^H S a) for a symmetric real CSC S on the plan's pattern."""
-        import ctypes as C  # noqa: F401
         from . import _lib as L
         n = colptr.numel() - 1
         dev = vals.device
