@@ -118,7 +118,9 @@ def transient_av_cpu(mesh, lib, sigma_e, source, time_steps, fixed_A, v_fixed_no
     sigma); reg = RegScaleRel * mean of the positive |diag J| at the first
     iteration of the first step; fixed DOFs: fixed_A, the given grounded
     nodes, else the first V DOF. The line search adds the regularisation
-    before zeroing the fixed rows (:222-226). Returns (solutions [N], Newton
+    before zeroing the fixed rows (:222-226). With linear materials only
+    (checkLinearity) there is no line search and a step stops after an update
+    from a residual below 1e-9 (:207, :240). Returns (solutions [N], Newton
     iteration counts, nA)."""
     from magpackage_amd.fem_tables import get_quadrature_data, lagrange_tet_p1, matlab_flat
     mi = O.mesh_inputs(mesh)
@@ -171,6 +173,7 @@ def transient_av_cpu(mesh, lib, sigma_e, source, time_steps, fixed_A, v_fixed_no
         R = R + Msig @ (x - x_prev) / dt - F + Cav @ x + (Cav.T @ (x - x_prev)) / dt + Lvv @ x
         return (csc(I, J, V) if calc_j else None), R
 
+    linear = all(getattr(m, "Type", "Linear") == "Linear" for m in lib.values())
     x_prev = np.zeros(N)
     reg = 0.0
     sols, iters = [], []
@@ -198,7 +201,7 @@ def transient_av_cpu(mesh, lib, sigma_e, source, time_steps, fixed_A, v_fixed_no
                 break
             dx = spla.splu(Jbc, permc_spec="MMD_AT_PLUS_A").solve(-Rbc)
             alpha = 1.0
-            if line_search:
+            if line_search and not linear:
                 for k in range(max_ls + 1):
                     a_try = alpha * 0.5 ** k
                     xt = x + a_try * dx
@@ -212,6 +215,8 @@ def transient_av_cpu(mesh, lib, sigma_e, source, time_steps, fixed_A, v_fixed_no
                     if rt > norm * 10:
                         alpha *= 0.1
             x = x + alpha * dx
+            if linear and norm < 1e-9:
+                break
         sols.append(x.copy())
         iters.append(it)
         x_prev = x
