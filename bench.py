@@ -46,13 +46,8 @@ def algorithmic_bytes_freq(nn, ne, nnz):
 
 
 def freq_workload(n=128, seed=2):
-    from magpackage_amd.material import MU0
-    from magpackage_amd.meshgen import kuhn_box, tag_slab
-    km = kuhn_box(n, n, n, seed=seed)
-    tag_slab(km, 2, 0.45, 0.55, 2)
-    nu = np.full(km.num_elems, 1.0 / MU0)
-    sigma = np.where(km.RegionTags == 2, 3.5e7, 0.0)
-    return km, nu, sigma, 2 * np.pi * 50.0
+    from magpackage_amd.workloads import plate_box
+    return plate_box(n, seed)
 
 
 def algorithmic_bytes_newton(nn, ne, n, nnz):
@@ -65,13 +60,8 @@ def newton_workload(nx=100, ny=100, nz=67, seed=3):
     """100x100x67-cube jittered Kuhn box (cube edge 0.01), an iron core
     (centroid in [0.3,0.7]x[0.3,0.7]x[0.2,0.47], Marrocco curve, sigma 2e6)
     in air (nu0 = 1/mu0)."""
-    from magpackage_amd.material import create_linear, marrocco
-    from magpackage_amd.meshgen import kuhn_box, tag_box
-    km = kuhn_box(nx, ny, nz, lengths=(nx / 100.0, ny / 100.0, nz / 100.0), seed=seed)
-    tag_box(km, (0.3, 0.3, 0.2), (0.7, 0.7, 0.47), 2)
-    lib = {1: create_linear(1.0), 2: marrocco()}
-    sigma = np.where(km.RegionTags == 2, 2e6, 0.0)
-    return km, lib, sigma
+    from magpackage_amd.workloads import newton_core_box
+    return newton_core_box(nx, ny, nz, seed)
 
 
 def cpu_baseline_newton(sample: int, reps: int):
@@ -158,9 +148,8 @@ def run_newton(args):
     n = plan.n
     dev = torch.device("cuda", 0)
     # A ~ N(0, 1) (seed 3), scaled so that max |B_e| = 2.2 T
-    a = torch.from_numpy(np.random.default_rng(3).normal(size=n)).to(dev)
-    bmax = float(asm.computeElementB(a, space).norm(dim=1).max())
-    a *= 2.2 / bmax
+    from magpackage_amd.workloads import newton_solution
+    a = newton_solution(asm, space, n, seed=3)
     bn = asm.computeElementB(a, space).norm(dim=1)
     log(f"[bench] plan {1e3 * (t3 - t2):.1f} ms: {n} dofs, nnz {plan.nnz}; "
         f"|B| in [{float(bn.min()):.3g}, {float(bn.max()):.3g}] T")
@@ -232,7 +221,7 @@ def run_ecore(args, emit=True):
     import torch
     from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
     from magpackage_amd.boundary import find_outer_boundary_dofs
-    from magpackage_amd.learned import NuPredictor, learned_warm_start
+    from magpackage_amd.learned import NuPredictor, learned_warm_start, training_collector
     from magpackage_amd.material import create_linear, marrocco
     from magpackage_amd.meshgen import ecore_box
     from magpackage_amd.transient import TransientSolver, divergence_free_source
@@ -273,23 +262,8 @@ def run_ecore(args, emit=True):
             pred = NuPredictor(asm, lib, seed=0, drive=lambda t: (np.sin(w * t), np.cos(w * t)))
             hook = learned_warm_start(pred, space, eta_warm=args.warm_eta)
             solver.InitialGuess = "extrapolated"  # extrapolation stays the fallback
-            bcache = {}
-
-            def on_step(ctx):
-                k = ctx["step"]
-                if k > args.train_steps:
-                    return
-                bp = bcache.get(k - 1)
-                if bp is None:
-                    bp = asm.computeElementB(ctx["x_prev"], space)
-                bp2 = bcache.get(k - 2, bp)
-                bx = asm.computeElementB(ctx["x"], space)
-                bcache[k] = bx
-                bcache.pop(k - 3, None)
-                pred.add_sample(bp, bp2, bx, ctx["t"])
-                if k == args.train_steps:
-                    info = pred.fit(iters=args.train_iters)
-                    log(f"[ecore] trained: {info}")
+            on_step = training_collector(pred, space, args.train_steps, args.train_iters,
+                                         log=log)
         t0 = time.perf_counter()
         _, infos = solver.solve(space, lib, sigma, src, steps, fixed, warm_start=hook,
                                 on_step=on_step)

@@ -152,6 +152,34 @@ def check(rc: int) -> None:
         raise MpkError(rc, msg.decode() if msg else "")
 
 
+class KrylovError(RuntimeError):
+    """A device Krylov solve (mpk_pcg_solve, mpk_pcg_aux_solve,
+    mpk_cocg_aux_solve) stopped at its iteration limit or on a non-finite
+    residual.  The C ABI reports both through its iteration count and
+    relative residual, not a status code; the reference's direct solves
+    (LinearSolver.m:23-24 checks NaN/Inf) have no such outcome, so a caller
+    standing in for one raises this instead of returning the unconverged x."""
+
+    def __init__(self, what: str, iterations: int, rel: float, tol: float):
+        self.iterations, self.rel_residual, self.tolerance = iterations, rel, tol
+        super().__init__(f"[MagPackage:Solver] {what} did not converge: relative residual "
+                         f"{rel:.3g} after {iterations} iterations (tolerance {tol:.3g})")
+
+
+def check_krylov(iters, rel, tol: float, what: str = "Krylov solve",
+                 finite_only: bool = False) -> tuple[int, float]:
+    """Raises KrylovError when the solve's relative residual is not finite or,
+    unless ``finite_only`` (inexact Newton steps, whose outer residual judges
+    them), above ``tol``.  ``iters`` / ``rel`` are ctypes values or numbers.
+    Returns (iterations, rel)."""
+    import math
+    it = int(getattr(iters, "value", iters))
+    r = float(getattr(rel, "value", rel))
+    if not math.isfinite(r) or (not finite_only and r > tol):
+        raise KrylovError(what, it, r, tol)
+    return it, r
+
+
 def ptr(t) -> int | None:
     """Device/host address of a torch tensor (None for None)."""
     if t is None:

@@ -189,19 +189,31 @@ class BatchAssembler:
         return np.asarray(out)
 
 
-def log_uniform_meshes(count: int = 256, lo: float = 5e3, hi: float = 2e5, seed: int = 4):
-    """Kuhn boxes whose tet counts are log-uniform on [lo, hi] (seeded): each
-    box is nx x ny x nz cubes (6 tets per cube) close to a cube in shape."""
-    from .meshgen import kuhn_box, tag_slab
+def log_uniform_shapes(count: int = 256, lo: float = 5e3, hi: float = 2e5, seed: int = 4):
+    """Box shapes (nx, ny, nz) and mesh seeds of the log-uniform batch: tet
+    counts log-uniform on [lo, hi] (seeded), each box nx x ny x nz cubes
+    (6 tets per cube) close to a cube in shape."""
     rng = np.random.default_rng(seed)
     targets = np.exp(rng.uniform(np.log(lo), np.log(hi), count))
-    meshes = []
+    out = []
     for i, ne in enumerate(targets):
         c = max(ne / 6.0, 1.0)
         nx = max(int(round(c ** (1.0 / 3.0))), 2)
         ny = max(int(round((c / nx) ** 0.5)), 2)
         nz = max(int(round(c / (nx * ny))), 2)
-        km = kuhn_box(nx, ny, nz, seed=seed * 1000 + i)
-        tag_slab(km, 2, 0.45, 0.55, 2)
-        meshes.append(km)
-    return meshes
+        out.append(((nx, ny, nz), seed * 1000 + i))
+    return out
+
+
+def log_uniform_mesh(shape, mesh_seed: int):
+    """One mesh of the log-uniform batch: a jittered Kuhn box with the
+    conducting slab (tag 2, centroid z in [0.45, 0.55])."""
+    from .meshgen import kuhn_box, tag_slab
+    km = kuhn_box(*shape, seed=mesh_seed)
+    tag_slab(km, 2, 0.45, 0.55, 2)
+    return km
+
+
+def log_uniform_meshes(count: int = 256, lo: float = 5e3, hi: float = 2e5, seed: int = 4):
+    """The log-uniform batch's meshes (log_uniform_shapes, log_uniform_mesh)."""
+    return [log_uniform_mesh(s, ms) for s, ms in log_uniform_shapes(count, lo, hi, seed)]

@@ -253,6 +253,8 @@ class TransientBDF2Solver(TransientSolver):
                                         self.LinearMaxIter, self.LinearCheckEvery,
                                         C.byref(it), C.byref(rel), self.sp))
         self._pcg_it += it.value
+        # the solve stands in for the reference's direct one: it must converge
+        L.check_krylov(it, rel, self.LinearRelTol, "circuit Newton step")
         return dx
 
     # -------------------------------------------------------------- solve

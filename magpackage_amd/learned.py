@@ -234,3 +234,32 @@ def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5,
         return xe + d
 
     return hook
+
+
+def training_collector(predictor: NuPredictor, space, train_steps: int, train_iters: int = 20000,
+                       log=None):
+    """on_step hook for TransientSolver.solve: after each of the first
+    ``train_steps`` steps it adds the step's (features, converged log nu)
+    samples to ``predictor``, and after the last of them trains the network
+    (the warm-start hook stays inactive until then)."""
+    asm = predictor.asm
+    bcache = {}
+
+    def on_step(ctx):
+        k = ctx["step"]
+        if k > train_steps:
+            return
+        bp = bcache.get(k - 1)
+        if bp is None:
+            bp = asm.computeElementB(ctx["x_prev"], space)
+        bp2 = bcache.get(k - 2, bp)
+        bx = asm.computeElementB(ctx["x"], space)
+        bcache[k] = bx
+        bcache.pop(k - 3, None)
+        predictor.add_sample(bp, bp2, bx, ctx["t"])
+        if k == train_steps:
+            info = predictor.fit(iters=train_iters)
+            if log is not None:
+                log(f"[learned] trained: {info}")
+
+    return on_step

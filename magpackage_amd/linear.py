@@ -106,6 +106,9 @@ class LinearSolver:
                                         C.byref(it), C.byref(rel), L.stream_ptr()))
         self.LastIterations = int(it.value)
         self.LastRelResidual = float(rel.value)
+        # an unconverged Krylov solve is an error, as a failed factorisation
+        # would be for the direct solvers this front end stands in for
+        L.check_krylov(it, rel, self.KrylovTolerance, "LinearSolver.solve")
         return x.cpu().numpy()
 
     # ------------------------------------------------------------------

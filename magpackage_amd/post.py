@@ -499,6 +499,8 @@ class ParameterExtractor:
                                             L.ptr(keep["work"]), self.LinearRelTol,
                                             self.LinearMaxIter, 10, C.byref(it), C.byref(rel),
                                             L.stream_ptr()))
+            L.check_krylov(it, rel, self.LinearRelTol,
+                           f"ParameterExtractor.extractMatrices (winding {k + 1})")
             self.Iterations.append(int(it.value))
             A[:, k] = x
         Lm = (Ct.T @ A).cpu().numpy()

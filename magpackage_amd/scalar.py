@@ -66,6 +66,7 @@ class ScalarSolver:
                                     C.byref(it), C.byref(rel), sp))
         torch.cuda.synchronize()
         t_solve = time.perf_counter() - t0
+        L.check_krylov(it, rel, float(self.LinearRelTol), "ScalarSolver.solve")
         x = torch.where(fixed_d.bool(), vfix_d, x)
         info = dict(SolveTime=t_solve, Iterations=int(it.value), RelResidual=float(rel.value))
         return x, info

@@ -379,6 +379,9 @@ class TransientSolver:
             rel_ref = None
             extrap = self.InitialGuess == "extrapolated" and step > 2
             if extrap and warm_start is None:
+                # the relative test stays the one plain Newton applies: against
+                # the residual at x_prev (the reference's first iterate, :159)
+                _, rel_ref = self.residual(x_prev, b, dt)
                 x, label = 2.0 * x_prev - x_prev2, "extrapolated"
             if warm_start is not None and self.reg_abs > 0:
                 # the residual decides between warm start, extrapolation or x_prev
@@ -568,6 +571,8 @@ class TransientSolver:
                     initial = norm_res if norm_res >= 1e-20 else 1.0
                 info.newton_iters = it
                 info.residual, info.rel_residual = norm_res, norm_res / initial
+                if not math.isfinite(norm_res):
+                    break  # a NaN/Inf residual ends the step unconverged
                 if norm_res < self.Tolerance or norm_res / initial < self.RelTolerance:
                     info.converged = True
                     break
@@ -595,6 +600,8 @@ class TransientSolver:
                                                 C.byref(itc), C.byref(rel), self.sp))
                 info.pcg_iters += int(itc.value)
                 info.pcg_per_solve.append(int(itc.value))
+                if not bool(torch.isfinite(w).all()):
+                    break  # the reference stops on a NaN step (TransientSolver.m:201-204)
                 dxv = w.clone()
                 dxv[nA:] = w[nA:] / dt       # V = W / dt
                 alpha = self.Relaxation

@@ -64,3 +64,20 @@ def test_rejections(gpu):
         ls.solve(U, np.ones(n))
     with pytest.raises(ValueError, match="Unknown method"):
         LinearSolver("LU").solve(A, np.ones(n))
+
+
+def test_unconverged_solve_raises(gpu):
+    """A Krylov solve stopped by KrylovMaxIter is an error (the direct solves
+    this front end stands in for cannot end unconverged): LinearSolver.solve
+    raises KrylovError instead of returning the partial x."""
+    from magpackage_amd._lib import KrylovError
+    from magpackage_amd.linear import LinearSolver
+    A, Ms, n = _matrices()
+    ls = LinearSolver()
+    ls.KrylovMaxIter = 5
+    b = np.random.default_rng(2).standard_normal(n)
+    with pytest.raises(KrylovError, match="did not converge") as ei:
+        ls.solve(A, b)
+    assert ei.value.iterations == 5 and ei.value.rel_residual > ls.KrylovTolerance
+    with pytest.raises(KrylovError):
+        ls.solve((A + 1j * 2 * np.pi * 50 * Ms).tocsc(), b + 0j)

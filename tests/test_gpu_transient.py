@@ -1,5 +1,7 @@
 """GPU transient Newton driver against its CPU restatement (oracle kernels +
-scipy direct solves): same converged fields within the solver tolerance."""
+scipy direct solves): ten strongly saturating backward-Euler steps with the
+GPU arm's linear systems solved to 1e-12 (a direct solve's accuracy), the
+same Newton iteration counts, and B within 1e-6 of the restatement's."""
 import numpy as np
 import pytest
 
@@ -23,7 +25,7 @@ def test_transient_matches_cpu_restatement(gpu):
     lib = {1: create_linear(1.0), 2: marrocco(), 3: create_linear(1.0)}
     sigma = np.where(m.RegionTags == 2, 2e6, 0.0)
     amp = 3e7
-    steps = [1e-3] * 3
+    steps = [1e-3] * 10
     src = lambda t: ec.js_dir * amp * np.sin(2 * np.pi * 50 * t)
     edges, _, _ = O.generate_edges(m.T)
     fixed_cpu = outer_boundary_edge_mask(m.T, edges)
@@ -37,6 +39,8 @@ def test_transient_matches_cpu_restatement(gpu):
     fixed = find_outer_boundary_dofs(M, dh, space)
     assert np.array_equal(fixed, fixed_cpu)
     solver = TransientSolver(asm)
+    # every Newton step's PCG to 1e-12, as the restatement's direct solves
+    solver.LinearEtaMax, solver.LinearGamma, solver.LinearEtaFloor = 1e-12, 1e30, 0.0
     js = torch.from_numpy(ec.js_dir).cuda()
     sols, infos = solver.solve(space, lib, {2: 2e6}, lambda t: js * (amp * np.sin(2 * np.pi * 50 * t)),
                                steps, fixed)
@@ -46,10 +50,12 @@ def test_transient_matches_cpu_restatement(gpu):
         print(f"step {k + 1}: gpu {info.newton_iters} Newton / {info.pcg_iters} PCG, "
               f"cpu {ref_iters[k]} Newton, rel res {info.rel_residual:.2e}", flush=True)
         assert info.converged
+        assert info.newton_iters == ref_iters[k]
         Bg = asm.computeElementB(x, space).cpu().numpy()
         Br = O.element_b(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], xr, curl)
         err = np.abs(Bg - Br).max() / np.abs(Br).max()
-        assert err < 1e-4, (k, err)
+        print(f"    B error {err:.1e}", flush=True)
+        assert err <= 1e-6, (k, err)
         assert np.all(x.cpu().numpy()[fixed] == 0.0)
 
 

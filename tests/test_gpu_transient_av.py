@@ -109,3 +109,34 @@ def test_transient_av_linear_materials(gpu):
         # 1e-12, amplified by the conditioning of the mu_r 1000 / sigma 5e6
         # system) is not reduced by later iterations: measured B 2e-7, V 2e-6
         assert eb <= 1e-6 and ev <= 1e-5
+
+
+def test_transient_av_nan_source_stops(gpu):
+    """A non-finite residual ends a solveAV step at once, unconverged, as the
+    reference breaks on a NaN Newton step (TransientSolver.m:201-204), rather
+    than running MaxIterations Newton iterations and line searches on NaNs."""
+    import torch
+    from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
+    from magpackage_amd.material import create_linear, marrocco
+    from magpackage_amd.meshgen import ecore_box
+    from magpackage_amd.transient import TransientSolver
+    em = ecore_box(4, 4, 3)
+    km = em.mesh
+    fixed = outer_fixed_mask(km)
+    mesh = Mesh.from_kuhn(km)
+    dh = DofHandler(mesh)
+    sA = FunctionSpace("Nedelec", 1)
+    dh.distributeDofs(sA)
+    solver = TransientSolver(Assembler(mesh, dh))
+    js = torch.from_numpy(em.js_dir * 3e6).cuda()
+    # a NaN initial A: every residual entry it reaches is NaN (the B-H lookup
+    # of a NaN |B|^2 stays on its first piece, so no table index leaves range)
+    nan_a = np.full(dh.NumGlobalDofs, np.nan)
+    _, infos = solver.solveAV(sA, FunctionSpace("Lagrange", 1),
+                              {1: create_linear(1.0), 2: marrocco(), 3: create_linear(1.0)},
+                              {1: 0.0, 2: 2e6, 3: 0.0}, lambda t: js, [1e-3], fixed,
+                              init_A=nan_a)
+    assert len(infos) == 1
+    assert not infos[0].converged
+    assert infos[0].newton_iters == 1 and infos[0].line_search_evals == 0
+    assert not np.isfinite(infos[0].residual)

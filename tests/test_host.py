@@ -182,3 +182,18 @@ def test_outer_boundary_edges_match_oracle():
     got = outer_boundary_edge_mask(T2, edges2)
     assert np.array_equal(got, O.boundary_edges(T2, edges2))
     assert got.sum() > O.boundary_edges(m.T, edges).sum()
+
+
+def test_check_krylov():
+    """The solve checker: a residual above the tolerance or non-finite raises,
+    unless only finiteness is asked for (inexact Newton steps)."""
+    import ctypes as C
+    from magpackage_amd._lib import KrylovError, check_krylov
+    assert check_krylov(C.c_int(12), C.c_double(1e-13), 1e-12) == (12, 1e-13)
+    with pytest.raises(KrylovError, match="after 40 iterations"):
+        check_krylov(40, 2e-12, 1e-12, "x")
+    with pytest.raises(KrylovError):
+        check_krylov(3, float("nan"), 1e-12)
+    assert check_krylov(40, 2e-3, 1e-12, finite_only=True) == (40, 2e-3)
+    with pytest.raises(KrylovError):
+        check_krylov(3, float("inf"), 1e-12, finite_only=True)
