@@ -299,6 +299,27 @@ int mpk_csc_newton_set(const mpk_plan* plan, const double* P, const int32_t* tag
                        int precision, double* k_vals, double* m_vals, double* j_vals,
                        double* residual, void* stream);
 
+/* Bytes of the library's static device scratch (one copy per device, held
+ * until the process ends) that mpk_csc_newton_set_noalloc works in. */
+#define MPK_NOALLOC_SCRATCH_BYTES (128ull << 20)
+
+/* mpk_csc_newton_set without any device allocation: no counterpart in the
+ * reference.  For callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Same
+ * arguments, outputs and values as mpk_csc_newton_set; the element state and
+ * tile classes live in the static scratch of MPK_NOALLOC_SCRATCH_BYTES
+ * instead of the plan's lazily allocated buffers, and the two tile passes run
+ * one after the other on `stream`.  Limit: 48 B per element (FP64) or 24 B
+ * (FP32_LOCAL) + 8 B per 64 columns must fit the scratch (about 2.7M / 5.5M
+ * elements); larger plans get MPK_ERR_LIMIT.  The current device must be the
+ * plan's.  Every call on a device shares that device's scratch: calls that
+ * overlap on one device (other streams, other threads) race. */
+int mpk_csc_newton_set_noalloc(const mpk_plan* plan, const double* P, const int32_t* tags,
+                               const double* sol_a, const mpk_material* mat,
+                               const double* sigma, const mpk_quadrature* quad_k,
+                               const mpk_quadrature* quad_m, int precision, double* k_vals,
+                               double* m_vals, double* j_vals, double* residual, void* stream);
+
 /* Source vector F [ndofs] = sparse(I(valid), 1, V(valid), N, 1) of
  * assemble_source_kernel_mex (src/Assembler.m:78-93), summed per DOF in
  * triplet order (bit-identical); one lane per DOF, no atomics.  Js [3 x ne]

@@ -112,6 +112,8 @@ _SIGS = {
     "mpk_csc_jacobian": ([_plan_p, _vp, _vp, _vp, _pm, _pq, _i32, _i32, _vp, _vp, _vp], _i32),
     "mpk_csc_newton_set": ([_plan_p, _vp, _vp, _vp, _pm, _vp, _pq, _pq, _i32, _vp, _vp, _vp,
                             _vp, _vp], _i32),
+    "mpk_csc_newton_set_noalloc": ([_plan_p, _vp, _vp, _vp, _pm, _vp, _pq, _pq, _i32, _vp, _vp,
+                                    _vp, _vp, _vp], _i32),
     "mpk_csc_drop_zeros": ([_plan_p, _vp, _i32, _vp, _vp, _vp, _pi64, _vp], _i32),
     "mpk_spmv_sym": ([_vp, _vp, _vp, _i64, _vp, _vp, _vp], _i32),
     "mpk_csc_combine": ([_i64, _d, _vp, _d, _vp, _d, _vp, _d, _vp, _vp, _vp], _i32),

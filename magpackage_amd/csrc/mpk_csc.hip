@@ -19,6 +19,7 @@
 //               FP64 geometry and accumulation, <= 1e-6 relative.
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "mpk_csc_exact.hpp"
@@ -187,14 +188,16 @@ k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
 }
 
 // Loads P (or checks that the plan holds coordinates), makes sure the state
-// buffer exists and fills it.  The buffer is allocated on the first call.
-// With tile_flag ([2 ntiles + 1] int32: class flags, class-1 tile list,
-// class-1 count), the tile classes and the list of class-1 tiles are written
-// too: flag_sigma / flag_dnu say which terms (M / D) classify a tile.
+// buffer exists and fills it.  The buffer is the plan's, allocated on the
+// first call, unless state_buf (ne x sizeof(S0) + ne x sizeof(S1) bytes) is
+// given.  With tile_flag ([2 ntiles + 1] int32: class flags, class-1 tile
+// list, class-1 count), the tile classes and the list of class-1 tiles are
+// written too: flag_sigma / flag_dnu say which terms (M / D) classify a tile.
 template <class SV>
 int elem_state(const mpk_plan* plan, const double* P, const int32_t* tags, const double* sol,
                const double* sigma, const mpk_material* mat, hipStream_t st, SV* sv,
-               int32_t* tile_flag = nullptr, bool flag_sigma = false, bool flag_dnu = false) {
+               int32_t* tile_flag = nullptr, bool flag_sigma = false, bool flag_dnu = false,
+               void* state_buf = nullptr) {
     using S0 = typename SV::S0;
     using S1 = typename SV::S1;
     if (P) {
@@ -205,9 +208,12 @@ int elem_state(const mpk_plan* plan, const double* P, const int32_t* tags, const
     }
     if (plan->ne == 0) return MPK_OK;
     MPK_REQUIRE(plan->cell, MPK_ERR_ARGS, "plan without cell DOFs");
-    // sized for the FP64 state; the FP32 one uses the first half
-    if (!plan->buf_state.p) MPK_HIP_TRY(plan->buf_state.alloc((size_t)plan->ne * 48));
-    S0* s0 = plan->buf_state.as<S0>();
+    S0* s0 = static_cast<S0*>(state_buf);
+    if (!s0) {
+        // sized for the FP64 state; the FP32 one uses the first half
+        if (!plan->buf_state.p) MPK_HIP_TRY(plan->buf_state.alloc((size_t)plan->ne * 48));
+        s0 = plan->buf_state.as<S0>();
+    }
     S1* s1 = reinterpret_cast<S1*>(s0 + plan->ne);
     const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
     if (tile_flag)
@@ -385,19 +391,30 @@ constexpr size_t kSidePerCu = 2;
 // the column tiles; tiles that no M or D term reaches run FrameNewtonLin (one
 // LDS plane), the others FrameNewton (two or three planes), in two launches.
 // Each tile is written by exactly one of them.
+// With scratch_state / scratch_flags given (the allocation-free entry point),
+// the element state and the tile classes live there instead of in the plan's
+// buffers, and the two passes run one after the other on the caller's stream
+// (the side-by-side schedule would create the plan's side stream).
 template <int kPrec, class SV>
 int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const double* sol_a,
                const mpk_material* mat, const double* sigma, const CscQuad& qk,
                const CscQuad& qm, double* k_vals, double* m_vals, double* j_vals,
-               double* residual, hipStream_t st) {
+               double* residual, hipStream_t st, void* scratch_state = nullptr,
+               int32_t* scratch_flags = nullptr) {
     const bool with_m = m_vals != nullptr;
     const bool split = plan->n > 0 && ring_driver(plan);
+    const bool noalloc = scratch_state != nullptr;
     int32_t* flags = nullptr;
     const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
     if (split) {
-        if (!plan->buf_tile_flag.p)
-            MPK_HIP_TRY(plan->buf_tile_flag.alloc((size_t)(2 * ntiles + 1) * sizeof(int32_t)));
-        flags = plan->buf_tile_flag.as<int32_t>();
+        if (noalloc) {
+            flags = scratch_flags;
+        } else {
+            if (!plan->buf_tile_flag.p)
+                MPK_HIP_TRY(
+                    plan->buf_tile_flag.alloc((size_t)(2 * ntiles + 1) * sizeof(int32_t)));
+            flags = plan->buf_tile_flag.as<int32_t>();
+        }
     }
     // The classes depend on sigma and dnu, not on which outputs are requested
     // (a call for a subset of the outputs runs each tile with the same op as
@@ -405,7 +422,7 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     // every tile.
     SV sv{};
     if (int rc = elem_state(plan, P, tags, sol_a, sigma, mat, st, &sv, flags,
-                            with_m || sigma != nullptr, true))
+                            with_m || sigma != nullptr, true, scratch_state))
         return rc;
     if (split) {
         const int32_t* list = flags + ntiles;
@@ -422,6 +439,10 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
         if (const char* e = std::getenv("MPK_LIST_BLOCKS_PER_CU")) {
             const int v = std::atoi(e);
             if (v >= 1 && v <= 16) per_cu = (size_t)v;
+        }
+        if (noalloc) {
+            overlap = false;
+            per_cu = 16;
         }
         auto listed = [&](hipStream_t s) {
             return with_m ? launch_ring_list(plan,
@@ -472,6 +493,36 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
                         FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr, j_vals, residual, qk,
                                                       qm, 0, 0},
                         2 * sizeof(double), 0, st, flags, 1);
+}
+
+// Scratch of the allocation-free Newton set (mpk_csc_newton_set_noalloc):
+// the element state and the tile classes of one call.  Each device that loads
+// the library holds its own copy until the process ends.  In host code the
+// name is a stand-in, not a device address: only the address that
+// hipGetSymbolAddress returns for the current device is passed to kernels.
+constexpr size_t kNoAllocBytes = (size_t)MPK_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (on the first call there) and kept.
+int scratch_address(void** out) {
+    int dev = -1;
+    MPK_HIP_TRY(hipGetDevice(&dev));
+    MPK_REQUIRE(dev >= 0 && dev < kMaxDevices, MPK_ERR_LIMIT, "device %d beyond the %d supported",
+                dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        MPK_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        MPK_REQUIRE(p != nullptr, MPK_ERR_HIP, "no device address for the scratch array");
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return MPK_OK;
 }
 
 }  // namespace
@@ -637,6 +688,50 @@ int mpk_csc_newton_set(const mpk_plan* plan, const double* P, const int32_t* tag
                                                     k_vals, m_vals, j_vals, residual, st);
     return newton_set<MPK_PREC_FP32_LOCAL, StateView32>(plan, P, tags, sol_a, mat, sigma, qk, qm,
                                                         k_vals, m_vals, j_vals, residual, st);
+}
+
+int mpk_csc_newton_set_noalloc(const mpk_plan* plan, const double* P, const int32_t* tags,
+                               const double* sol_a, const mpk_material* mat,
+                               const double* sigma, const mpk_quadrature* quad_k,
+                               const mpk_quadrature* quad_m, int precision, double* k_vals,
+                               double* m_vals, double* j_vals, double* residual, void* stream) {
+    clear_error();
+    MPK_REQUIRE(plan && mat, MPK_ERR_ARGS, "NULL plan or material");
+    MPK_REQUIRE(plan->n == 0 || (tags && sol_a), MPK_ERR_ARGS, "NULL array argument");
+    MPK_REQUIRE(precision == MPK_PREC_FP64 || precision == MPK_PREC_FP32_LOCAL, MPK_ERR_ARGS,
+                "the fused Newton set has FP64 and FP32_LOCAL modes only (got %d)", precision);
+    CscQuad qk, qm;
+    if (int rc = make_csc_quad(quad_k, true, qk)) return rc;
+    if (int rc = make_csc_quad(quad_m, false, qm)) return rc;
+    MPK_REQUIRE(standard_curl_table(quad_k), MPK_ERR_ARGS,
+                "the fused Newton set needs the Nedelec P1 reference curls");
+    MPK_REQUIRE(mat->num_tags >= 1, MPK_ERR_ARGS, "material table without tags");
+    // scratch layout: element state (48 B per element in FP64, 24 B in
+    // FP32_LOCAL) at offset 0, then the tile classes at a 256-byte boundary
+    const size_t per_elem = precision == MPK_PREC_FP64 ? 48 : 24;
+    const size_t state_bytes = ((size_t)plan->ne * per_elem + 255) & ~(size_t)255;
+    const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
+    const size_t need = state_bytes + (size_t)(2 * ntiles + 1) * sizeof(int32_t);
+    MPK_REQUIRE(need <= kNoAllocBytes, MPK_ERR_LIMIT,
+                "the allocation-free Newton set needs %zu B of scratch for %lld elements and "
+                "%lld columns; its scratch holds %zu B (use mpk_csc_newton_set)",
+                need, (long long)plan->ne, (long long)plan->n, kNoAllocBytes);
+    int dev = -1;
+    MPK_HIP_TRY(hipGetDevice(&dev));
+    MPK_REQUIRE(dev == plan->device, MPK_ERR_ARGS,
+                "the plan lives on device %d but device %d is current", plan->device, dev);
+    void* base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    unsigned char* sb = static_cast<unsigned char*>(base);
+    int32_t* flags = reinterpret_cast<int32_t*>(sb + state_bytes);
+    hipStream_t st = as_stream(stream);
+    if (precision == MPK_PREC_FP64)
+        return newton_set<MPK_PREC_FP64, StateView>(plan, P, tags, sol_a, mat, sigma, qk, qm,
+                                                    k_vals, m_vals, j_vals, residual, st, sb,
+                                                    flags);
+    return newton_set<MPK_PREC_FP32_LOCAL, StateView32>(plan, P, tags, sol_a, mat, sigma, qk, qm,
+                                                        k_vals, m_vals, j_vals, residual, st, sb,
+                                                        flags);
 }
 
 }  // extern "C"
