@@ -241,7 +241,9 @@ int mpk_plan_info(const mpk_plan* plan, int64_t* nnz, int64_t* ncols, int64_t* n
                   int64_t* n_incidences, int64_t* max_tile_nnz);
 /* *ring != 0 when the closed-form (FP64, FP32_LOCAL) kernels walk each
  * column's elements in rotation order around the edge (every column a simple
- * ring or chain, < 2^24 elements and nodes); else element order. */
+ * ring or chain, < 2^24 elements and nodes); else element order.  *ring == 2:
+ * the walk reads the compact 8-byte ring records (new node within 2^21 ids
+ * of the column's lower node, columns of <= 63 entries); 1: 16-byte ones. */
 int mpk_plan_ring_order(const mpk_plan* plan, int* ring);
 /* Device pointers owned by the plan: colptr [n+1], rowind [nnz] (0-based). */
 int mpk_plan_pattern(const mpk_plan* plan, const int32_t** colptr, const int32_t** rowind);

@@ -223,6 +223,8 @@ class Plan:
         ring = C.c_int()
         L.check(L.lib.mpk_plan_ring_order(self._p, C.byref(ring)))
         self.ring_order = bool(ring.value)
+        # ring record format: "compact" (8 B per incidence), "16" or None
+        self.ring_format = {0: None, 1: "16", 2: "compact"}[int(ring.value)]
         cp, ri = C.c_void_p(), C.c_void_p()
         L.check(L.lib.mpk_plan_pattern(self._p, C.byref(cp), C.byref(ri)))
         self._colptr_addr, self._rowind_addr = cp.value, ri.value

@@ -37,7 +37,7 @@ struct BatchShared {
     int acc_bytes_max;
 };
 
-template <int kPrec, int kB>
+template <int kPrec, int kB, bool kC>
 __global__ void __launch_bounds__(kW)
 k_csc_batch(const BatchMesh* __restrict__ meshes, const int2* __restrict__ tiles, int64_t ntiles,
             const double* __restrict__ omegas, BatchShared sh, int64_t chunk) {
@@ -57,7 +57,7 @@ k_csc_batch(const BatchMesh* __restrict__ meshes, const int2* __restrict__ tiles
         const BatchMesh& bm = meshes[mt.x];
         FrameKMF<kPrec> op{FrameKM<kPrec>{bm.nu, bm.sigma, bm.out, 0.0, sh.qk, sh.qm, 0},
                            omegas, sh.nfreq, bm.nnz, sh.acc_bytes_max};
-        ring_tile<FrameKMF<kPrec>, kB>(bm.tc, bm.ends, op, smem, mt.y);
+        ring_tile<FrameKMF<kPrec>, kB, kC>(bm.tc, bm.ends, op, smem, mt.y);
         __syncthreads();
     }
 }
@@ -70,6 +70,7 @@ struct mpk_batch {
     int precision = MPK_PREC_FP64;
     int ring_batch = 3;
     int blocks_per_cu = 0;
+    bool compact = false;  // every plan has compact ring records
     int num_cus = 256;
     size_t lds = 0;
     mpk::BatchShared sh{};
@@ -113,9 +114,16 @@ int mpk_batch_create(const mpk_batch_item* items, int64_t count, const double* o
             return fail(MPK_ERR_ARGS);
         }
         BatchMesh& bm = hm[(size_t)m];
-        bm.tc = TileCtx{p->colptr,   p->inc_ptr,   p->inc_slot, p->inc_pos,
+        if (m > 0 && p->ring8 != b->compact) {
+            set_error("batch item %lld: the plans of one batch must share a ring record format",
+                      (long long)m);
+            return fail(MPK_ERR_ARGS);
+        }
+        b->compact = p->ring8;
+        bm.tc = TileCtx{p->colptr,   p->inc_ptr,   p->inc_slot,   p->inc_pos,
                         p->inc_frame, p->inc_ring, p->frame_base, p->T,
-                        p->P4,        p->rowind,   p->n};
+                        p->P4,        p->rowind,   p->n,          p->inc_ring8,
+                        p->col_head};
         bm.ends = p->col_ends;
         bm.nu = items[m].nu;
         bm.sigma = items[m].sigma;
@@ -198,8 +206,14 @@ int mpk_batch_run(const mpk_batch* b, void* stream) {
     const int2* ts = b->tiles.as<int2>();
     const double* om = b->omegas.as<double>();
 #define MPK_BATCH_LAUNCH(PREC, KB)                                                            \
-    hipLaunchKernelGGL((k_csc_batch<PREC, KB>), dim3(grid), dim3(kW), b->lds, st, ms, ts,     \
-                       b->ntiles, om, b->sh, chunk)
+    do {                                                                                      \
+        if (b->compact)                                                                       \
+            hipLaunchKernelGGL((k_csc_batch<PREC, KB, true>), dim3(grid), dim3(kW), b->lds,  \
+                               st, ms, ts, b->ntiles, om, b->sh, chunk);                      \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_csc_batch<PREC, KB, false>), dim3(grid), dim3(kW), b->lds, \
+                               st, ms, ts, b->ntiles, om, b->sh, chunk);                      \
+    } while (0)
     if (b->precision == MPK_PREC_FP64) {
         if (b->ring_batch == 2) MPK_BATCH_LAUNCH(MPK_PREC_FP64, 2);
         else if (b->ring_batch == 4) MPK_BATCH_LAUNCH(MPK_PREC_FP64, 4);

@@ -49,9 +49,10 @@ bool standard_curl_table(const mpk_quadrature* q) {
 }
 
 TileCtx tile_ctx(const mpk_plan* plan) {
-    return TileCtx{plan->colptr,   plan->inc_ptr,   plan->inc_slot, plan->inc_pos,
+    return TileCtx{plan->colptr,   plan->inc_ptr,   plan->inc_slot,   plan->inc_pos,
                    plan->inc_frame, plan->inc_ring, plan->frame_base, plan->T,
-                   plan->P4,        plan->rowind,   plan->n};
+                   plan->P4,        plan->rowind,   plan->n,          plan->inc_ring8,
+                   plan->col_head};
 }
 
 template <class Op>
@@ -92,6 +93,18 @@ __device__ inline double2 put_state(float4* s0, float2* s1, int64_t e, double bx
     return make_double2(v.x, v.y);
 }
 
+// Zeroes n doubles with streaming (non-temporal) stores: 1024 per block,
+// coalesced 64-lane runs.
+constexpr int kFillBlock = 256;
+__global__ void __launch_bounds__(kFillBlock) k_fill_zero(double* __restrict__ p, int64_t n) {
+    const int64_t b = (int64_t)blockIdx.x * (4 * kFillBlock) + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = b + (int64_t)k * kFillBlock;
+        if (i < n) st_out(p + i, 0.0);
+    }
+}
+
 // Clears the tile class flags and the class-1 tile count.
 __global__ void k_clear_tiles(int32_t* __restrict__ flag, int64_t ntiles,
                               int32_t* __restrict__ count) {
@@ -127,10 +140,18 @@ k_tile_list(const int32_t* __restrict__ flag, int64_t ntiles, int32_t* __restric
     if (on) list[off + before] = (int32_t)t;
 }
 
+// Elements [e_begin, ne) of the mesh.  kStateElems elements per thread:
+// every independent load of all of them
+// (connectivity, cell DOFs, tags, sigma) goes out first, then the node and
+// solution gathers of all of them, then the arithmetic, so each wave keeps
+// twice the loads in flight of a one-element thread.
+constexpr int kStateElems = 2;
+
 template <class S0, class S1>
 __global__ void __launch_bounds__(kStateBlock)
 k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
-             const double4* __restrict__ P4, int64_t ne, const double* __restrict__ sol,
+             const double4* __restrict__ P4, int64_t e_begin, int64_t ne,
+             const double* __restrict__ sol,
              const int32_t* __restrict__ tags, const double* __restrict__ sigma, MaterialView mat,
              int ntags, int nbrk, int ncoe, S0* __restrict__ s0, S1* __restrict__ s1,
              int32_t* __restrict__ tile_flag, int flag_sigma, int flag_dnu) {
@@ -144,60 +165,131 @@ k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
         coe = tab + nbrk;
     }
     __syncthreads();
-    const int64_t e = (int64_t)blockIdx.x * kStateBlock + threadIdx.x;
-    if (e >= ne) return;
-    const int4 t = T[e];
-    const int nd[4] = {t.x, t.y, t.z, t.w};
-    const Vec3 x0 = ld_node(P4, t.x), x1 = ld_node(P4, t.y), x2 = ld_node(P4, t.z),
-               x3 = ld_node(P4, t.w);
-    const Vec3 d1 = sub(x1, x0), d2 = sub(x2, x0), d3 = sub(x3, x0);
-    const double det = dot(d1, cross(d2, d3));
-    double al[6];
+    // XCD-contiguous element ranges: blocks are dealt round-robin over the
+    // 8 XCDs, and neighbouring elements share nodes and edges, so each XCD
+    // takes one contiguous range and its L2 keeps the shared data
+    const int64_t e0 =
+        e_begin + xcd_tile(blockIdx.x, gridDim.x) * (kStateBlock * kStateElems) + threadIdx.x;
+    if (e0 >= ne) return;
+    int64_t e[kStateElems];
+    bool ok[kStateElems];
+    int4 t[kStateElems];
+    int2 c[kStateElems][3];
+    int tg[kStateElems];
+    double sg[kStateElems];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        const int d = cell[6 * e + i];
-        const double s = nd[edge_a(i)] <= nd[edge_b(i)] ? 1.0 : -1.0;
-        al[i] = d >= 0 ? sol[d] * s : 0.0;
+    for (int k = 0; k < kStateElems; ++k) {
+        const int64_t ek = e0 + (int64_t)k * kStateBlock;
+        ok[k] = ek < ne;
+        e[k] = ok[k] ? ek : e0;  // a valid element for the loads
+        t[k] = T[e[k]];
+        const int2* cr = reinterpret_cast<const int2*>(cell + 6 * e[k]);
+        c[k][0] = cr[0];
+        c[k][1] = cr[1];
+        c[k][2] = cr[2];
+        tg[k] = tags[e[k]];
+        sg[k] = sigma ? sigma[e[k]] : 1.0;
     }
-    // v = sum_i al_i c_i, c = (0,-2,2) (2,0,-2) (-2,2,0) (0,0,2) (0,-2,0) (2,0,0)
-    const double vx = 2.0 * (al[1] - al[2] + al[5]);
-    const double vy = 2.0 * (al[2] - al[0] - al[4]);
-    const double vz = 2.0 * (al[0] - al[1] + al[3]);
-    const double inv = 1.0 / det;
-    const double bx = (d1.x * vx + d2.x * vy + d3.x * vz) * inv;
-    const double by = (d1.y * vx + d2.y * vy + d3.y * vz) * inv;
-    const double bz = (d1.z * vx + d2.z * vy + d3.z * vz) * inv;
-    int tag = tags[e];
-    // tags are validated by the host layer; clamp so a bad tag cannot read
-    // outside the material table
-    tag = tag < 1 ? 1 : (tag > ntags ? ntags : tag);
-    double nu, dnu;
-    eval_nu(bx * bx + by * by + bz * bz, tag, mat, brk, coe, nu, dnu);
-    const double2 ds = put_state(s0, s1, e, bx, by, bz, nu, dnu, sigma ? sigma[e] : 1.0);
-    // Tile classes for the split Newton set: an element with a D term (dnu != 0)
-    // or an M term (|sigma| > 1e-12), the tests FrameNewton applies, marks the
-    // column tiles of its 6 DOFs.  Every writer stores the same value, so plain
-    // stores need no atomics.
-    if (tile_flag && ((flag_dnu && ds.x != 0.0) || (flag_sigma && fabs(ds.y) > 1e-12))) {
+    Vec3 x[kStateElems][4];
+    double al[kStateElems][6];
+#pragma unroll
+    for (int k = 0; k < kStateElems; ++k) {
+        x[k][0] = ld_node(P4, t[k].x);
+        x[k][1] = ld_node(P4, t[k].y);
+        x[k][2] = ld_node(P4, t[k].z);
+        x[k][3] = ld_node(P4, t[k].w);
+        const int nd[4] = {t[k].x, t[k].y, t[k].z, t[k].w};
+        const int dof[6] = {c[k][0].x, c[k][0].y, c[k][1].x, c[k][1].y, c[k][2].x, c[k][2].y};
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-            const int d = cell[6 * e + i];
-            if (d >= 0) tile_flag[d / kTileCols] = 1;
+            const double sgn = nd[edge_a(i)] <= nd[edge_b(i)] ? 1.0 : -1.0;
+            al[k][i] = dof[i] >= 0 ? sol[dof[i]] * sgn : 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kStateElems; ++k) {
+        if (!ok[k]) continue;
+        const Vec3 d1 = sub(x[k][1], x[k][0]), d2 = sub(x[k][2], x[k][0]),
+                   d3 = sub(x[k][3], x[k][0]);
+        const double det = dot(d1, cross(d2, d3));
+        // v = sum_i al_i c_i, c = (0,-2,2) (2,0,-2) (-2,2,0) (0,0,2) (0,-2,0) (2,0,0)
+        const double* a = al[k];
+        const double vx = 2.0 * (a[1] - a[2] + a[5]);
+        const double vy = 2.0 * (a[2] - a[0] - a[4]);
+        const double vz = 2.0 * (a[0] - a[1] + a[3]);
+        const double inv = 1.0 / det;
+        const double bx = (d1.x * vx + d2.x * vy + d3.x * vz) * inv;
+        const double by = (d1.y * vx + d2.y * vy + d3.y * vz) * inv;
+        const double bz = (d1.z * vx + d2.z * vy + d3.z * vz) * inv;
+        // tags are validated by the host layer; clamp so a bad tag cannot
+        // read outside the material table
+        const int tag = tg[k] < 1 ? 1 : (tg[k] > ntags ? ntags : tg[k]);
+        double nu, dnu;
+        eval_nu(bx * bx + by * by + bz * bz, tag, mat, brk, coe, nu, dnu);
+        const double2 ds = put_state(s0, s1, e[k], bx, by, bz, nu, dnu, sg[k]);
+        // Tile classes for the split Newton set: an element with a D term
+        // (dnu != 0) or an M term (|sigma| > 1e-12), the tests FrameNewton
+        // applies, marks the column tiles of its 6 DOFs.  Every writer stores
+        // the same value, so plain stores need no atomics.
+        if (tile_flag && ((flag_dnu && ds.x != 0.0) || (flag_sigma && fabs(ds.y) > 1e-12))) {
+            const int dof[6] = {c[k][0].x, c[k][0].y, c[k][1].x, c[k][1].y, c[k][2].x, c[k][2].y};
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                if (dof[i] >= 0) tile_flag[dof[i] / kTileCols] = 1;
         }
     }
 }
 
-// Loads P (or checks that the plan holds coordinates), makes sure the state
-// buffer exists and fills it.  The buffer is the plan's, allocated on the
-// first call, unless state_buf (ne x sizeof(S0) + ne x sizeof(S1) bytes) is
-// given.  With tile_flag ([2 ntiles + 1] int32: class flags, class-1 tile
-// list, class-1 count), the tile classes and the list of class-1 tiles are
-// written too: flag_sigma / flag_dnu say which terms (M / D) classify a tile.
+// Launch context of the element-state kernel over element ranges, with the
+// tile-class clear before and the class-1 tile list after.
 template <class SV>
-int elem_state(const mpk_plan* plan, const double* P, const int32_t* tags, const double* sol,
-               const double* sigma, const mpk_material* mat, hipStream_t st, SV* sv,
-               int32_t* tile_flag = nullptr, bool flag_sigma = false, bool flag_dnu = false,
-               void* state_buf = nullptr) {
+struct ElemStateRun {
+    const mpk_plan* plan = nullptr;
+    const double* sol = nullptr;
+    const int32_t* tags = nullptr;
+    const double* sigma = nullptr;
+    MaterialView mv{};
+    int ntags = 0, nbrk = 0, ncoe = 0;
+    typename SV::S0* s0 = nullptr;
+    typename SV::S1* s1 = nullptr;
+    int32_t* flag = nullptr;
+    int flag_sigma = 0, flag_dnu = 0;
+    int64_t ntiles = 0;
+    hipError_t clear(hipStream_t st) const {
+        if (flag)
+            hipLaunchKernelGGL(k_clear_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, st,
+                               flag, ntiles, flag + 2 * ntiles);
+        return hipGetLastError();
+    }
+    hipError_t run(int64_t b, int64_t e, hipStream_t st) const {
+        if (e > b)
+            hipLaunchKernelGGL((k_elem_state<typename SV::S0, typename SV::S1>),
+                               dim3(grid_for(e - b, kStateBlock * kStateElems)),
+                               dim3(kStateBlock), 0, st, plan->T, plan->cell, plan->P4, b, e, sol,
+                               tags, sigma, mv, ntags, nbrk, ncoe, s0, s1, flag, flag_sigma,
+                               flag_dnu);
+        return hipGetLastError();
+    }
+    hipError_t list(hipStream_t st) const {
+        if (flag)
+            hipLaunchKernelGGL(k_tile_list, dim3(grid_for(ntiles, kListBlock)), dim3(kListBlock),
+                               0, st, flag, ntiles, flag + ntiles, flag + 2 * ntiles);
+        return hipGetLastError();
+    }
+};
+
+// Loads P (or checks that the plan holds coordinates), makes sure the state
+// buffer exists and fills in the launch context.  The buffer is the plan's,
+// allocated on the first call, unless state_buf (ne x sizeof(S0) + ne x
+// sizeof(S1) bytes) is given.  With tile_flag ([2 ntiles + 1] int32: class
+// flags, class-1 tile list, class-1 count), the tile classes and the list of
+// class-1 tiles are written too: flag_sigma / flag_dnu say which terms (M / D)
+// classify a tile.  run->plan stays NULL for a plan without elements.
+template <class SV>
+int elem_state_prepare(const mpk_plan* plan, const double* P, const int32_t* tags,
+                       const double* sol, const double* sigma, const mpk_material* mat,
+                       hipStream_t st, SV* sv, int32_t* tile_flag, bool flag_sigma,
+                       bool flag_dnu, void* state_buf, ElemStateRun<SV>* run) {
     using S0 = typename SV::S0;
     using S1 = typename SV::S1;
     if (P) {
@@ -215,24 +307,39 @@ int elem_state(const mpk_plan* plan, const double* P, const int32_t* tags, const
         s0 = plan->buf_state.as<S0>();
     }
     S1* s1 = reinterpret_cast<S1*>(s0 + plan->ne);
-    const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
-    if (tile_flag)
-        hipLaunchKernelGGL(k_clear_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, st,
-                           tile_flag, ntiles, tile_flag + 2 * ntiles);
-    MaterialView mv{mat->linear_nu, mat->is_nonlinear, mat->max_bsq,    mat->breaks,
-                    mat->coefs,     mat->spline_start, mat->spline_count, mat->coef_start};
-    hipLaunchKernelGGL((k_elem_state<S0, S1>), dim3(grid_for(plan->ne, kStateBlock)),
-                       dim3(kStateBlock), 0,
-                       st, plan->T, plan->cell, plan->P4, plan->ne, sol, tags, sigma, mv,
-                       (int)mat->num_tags, (int)mat->num_breaks, (int)mat->num_coefs, s0, s1,
-                       tile_flag, (int)flag_sigma, (int)flag_dnu);
-    MPK_HIP_TRY(hipGetLastError());
-    if (tile_flag) {
-        hipLaunchKernelGGL(k_tile_list, dim3(grid_for(ntiles, kListBlock)), dim3(kListBlock), 0,
-                           st, tile_flag, ntiles, tile_flag + ntiles, tile_flag + 2 * ntiles);
-        MPK_HIP_TRY(hipGetLastError());
-    }
+    run->plan = plan;
+    run->sol = sol;
+    run->tags = tags;
+    run->sigma = sigma;
+    run->mv = MaterialView{mat->linear_nu, mat->is_nonlinear, mat->max_bsq,    mat->breaks,
+                           mat->coefs,     mat->spline_start, mat->spline_count, mat->coef_start};
+    run->ntags = (int)mat->num_tags;
+    run->nbrk = (int)mat->num_breaks;
+    run->ncoe = (int)mat->num_coefs;
+    run->s0 = s0;
+    run->s1 = s1;
+    run->flag = tile_flag;
+    run->flag_sigma = (int)flag_sigma;
+    run->flag_dnu = (int)flag_dnu;
+    run->ntiles = (plan->n + kTileCols - 1) / kTileCols;
     *sv = SV{s0, s1};
+    return MPK_OK;
+}
+
+// The element state of the whole mesh on one stream: clear, state, list.
+template <class SV>
+int elem_state(const mpk_plan* plan, const double* P, const int32_t* tags, const double* sol,
+               const double* sigma, const mpk_material* mat, hipStream_t st, SV* sv,
+               int32_t* tile_flag = nullptr, bool flag_sigma = false, bool flag_dnu = false,
+               void* state_buf = nullptr) {
+    ElemStateRun<SV> r;
+    if (int rc = elem_state_prepare(plan, P, tags, sol, sigma, mat, st, sv, tile_flag,
+                                    flag_sigma, flag_dnu, state_buf, &r))
+        return rc;
+    if (!r.plan) return MPK_OK;
+    MPK_HIP_TRY(r.clear(st));
+    MPK_HIP_TRY(r.run(0, plan->ne, st));
+    MPK_HIP_TRY(r.list(st));
     return MPK_OK;
 }
 
@@ -289,6 +396,58 @@ bool ring_driver(const mpk_plan* plan) {
     return plan->ring && !(drv && std::strcmp(drv, "element") == 0);
 }
 
+// Launches of the ring kernels for a gather batch chosen at run time.
+template <class Op, bool kC>
+void launch_batch_k(int kb, unsigned grid, size_t lds, hipStream_t st, const TileCtx& tc,
+                    const int2* ends, const Op& op, const int32_t* sel, int32_t want) {
+    switch (kb) {
+        case 2:
+            hipLaunchKernelGGL((k_csc_ring_batch<Op, 2, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, sel, want);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_csc_ring_batch<Op, 4, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, sel, want);
+            break;
+        case 6:
+            hipLaunchKernelGGL((k_csc_ring_batch<Op, 6, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, sel, want);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_csc_ring_batch<Op, 8, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, sel, want);
+            break;
+        default:
+            hipLaunchKernelGGL((k_csc_ring_batch<Op, 3, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, sel, want);
+    }
+}
+template <class Op, bool kC>
+void launch_list_k(int kb, unsigned grid, size_t lds, hipStream_t st, const TileCtx& tc,
+                   const int2* ends, const Op& op, const int32_t* list, const int32_t* count) {
+    switch (kb) {
+        case 2:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 2, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, list, count);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 4, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, list, count);
+            break;
+        case 6:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 6, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, list, count);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 8, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, list, count);
+            break;
+        default:
+            hipLaunchKernelGGL((k_csc_ring_list<Op, 3, kC>), dim3(grid), dim3(kW), lds, st, tc,
+                               ends, op, list, count);
+    }
+}
+
 // Persistent launch of the ring driver over the tiles of a device list
 // (list[0, *count)): as many blocks as fit on the device at the op's LDS size.
 template <class Op>
@@ -313,27 +472,12 @@ int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t s
     g = (g + 7) / 8 * 8;
     const unsigned grid = (unsigned)(g > 8 ? g : 8);
     const TileCtx tc = tile_ctx(plan);
-    switch (ring_batch<Op>()) {
-        case 2:
-            hipLaunchKernelGGL((k_csc_ring_list<Op, 2>), dim3(grid), dim3(kW), lds, st, tc,
-                               plan->col_ends, op, list, count);
-            break;
-        case 4:
-            hipLaunchKernelGGL((k_csc_ring_list<Op, 4>), dim3(grid), dim3(kW), lds, st, tc,
-                               plan->col_ends, op, list, count);
-            break;
-        case 6:
-            hipLaunchKernelGGL((k_csc_ring_list<Op, 6>), dim3(grid), dim3(kW), lds, st, tc,
-                               plan->col_ends, op, list, count);
-            break;
-        case 8:
-            hipLaunchKernelGGL((k_csc_ring_list<Op, 8>), dim3(grid), dim3(kW), lds, st, tc,
-                               plan->col_ends, op, list, count);
-            break;
-        default:
-            hipLaunchKernelGGL((k_csc_ring_list<Op, 3>), dim3(grid), dim3(kW), lds, st, tc,
-                               plan->col_ends, op, list, count);
-    }
+    if (plan->ring8)
+        launch_list_k<Op, true>(ring_batch<Op>(), grid, lds, st, tc, plan->col_ends, op, list,
+                                count);
+    else
+        launch_list_k<Op, false>(ring_batch<Op>(), grid, lds, st, tc, plan->col_ends, op, list,
+                                 count);
     MPK_HIP_TRY(hipGetLastError());
     return MPK_OK;
 }
@@ -354,27 +498,12 @@ int launch_frame(const mpk_plan* plan, const double* P, Op op, size_t acc_size, 
     const unsigned grid = grid_for(plan->n, kW);
     if (ring_driver(plan)) {
         const TileCtx tc = tile_ctx(plan);
-        switch (ring_batch<Op>()) {
-            case 2:
-                hipLaunchKernelGGL((k_csc_ring_batch<Op, 2>), dim3(grid), dim3(kW), lds, st, tc,
-                                   plan->col_ends, op, sel, want);
-                break;
-            case 4:
-                hipLaunchKernelGGL((k_csc_ring_batch<Op, 4>), dim3(grid), dim3(kW), lds, st, tc,
-                                   plan->col_ends, op, sel, want);
-                break;
-            case 6:
-                hipLaunchKernelGGL((k_csc_ring_batch<Op, 6>), dim3(grid), dim3(kW), lds, st, tc,
-                                   plan->col_ends, op, sel, want);
-                break;
-            case 8:
-                hipLaunchKernelGGL((k_csc_ring_batch<Op, 8>), dim3(grid), dim3(kW), lds, st, tc,
-                                   plan->col_ends, op, sel, want);
-                break;
-            default:
-                hipLaunchKernelGGL((k_csc_ring_batch<Op, 3>), dim3(grid), dim3(kW), lds, st, tc,
-                                   plan->col_ends, op, sel, want);
-        }
+        if (plan->ring8)
+            launch_batch_k<Op, true>(ring_batch<Op>(), grid, lds, st, tc, plan->col_ends, op, sel,
+                                     want);
+        else
+            launch_batch_k<Op, false>(ring_batch<Op>(), grid, lds, st, tc, plan->col_ends, op,
+                                      sel, want);
     } else {
         MPK_REQUIRE(!sel, MPK_ERR_ARGS, "tile selection needs the ring driver");
         hipLaunchKernelGGL(k_csc_frame<Op>, dim3(grid), dim3(kW), lds, st, tile_ctx(plan), op);
@@ -415,6 +544,24 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
                     plan->buf_tile_flag.alloc((size_t)(2 * ntiles + 1) * sizeof(int32_t)));
             flags = plan->buf_tile_flag.as<int32_t>();
         }
+    }
+    // M is zero on the tiles no M term reaches.  A streaming fill on the
+    // plan's side stream writes zeros over the whole M array while the
+    // element state and the class-0 pass run; the class-0 pass then stores K
+    // and J only, and the listed pass, ordered after the fill, writes its
+    // tiles' M (4M-tet Newton workload: 0.738-0.747 -> 0.720 ms FP32_LOCAL,
+    // 0.775-0.780 -> 0.764-0.767 ms FP64).  MPK_NEWTON_MFILL=0 stores the
+    // zeros from the class-0 pass instead; the allocation-free entry point
+    // always does (it creates no stream).
+    bool mfill = split && with_m && !noalloc && plan->nnz > 0;
+    if (const char* e = std::getenv("MPK_NEWTON_MFILL")) mfill = mfill && std::atoi(e) != 0;
+    if (mfill) {
+        MPK_HIP_TRY(plan->side.ensure());
+        MPK_HIP_TRY(hipEventRecord(plan->side.fork, st));
+        MPK_HIP_TRY(hipStreamWaitEvent(plan->side.s, plan->side.fork, 0));
+        hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(plan->nnz, 4 * kFillBlock)),
+                           dim3(kFillBlock), 0, plan->side.s, m_vals, plan->nnz);
+        MPK_HIP_TRY(hipGetLastError());
     }
     // The classes depend on sigma and dnu, not on which outputs are requested
     // (a call for a subset of the outputs runs each tile with the same op as
@@ -458,12 +605,16 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
         };
         auto class0 = [&]() {
             return launch_frame(plan, nullptr,
-                                FrameNewtonLin<kPrec, SV>{sv, k_vals, m_vals, j_vals, residual, qk,
-                                                          0},
+                                FrameNewtonLin<kPrec, SV>{sv, k_vals, mfill ? nullptr : m_vals,
+                                                          j_vals, residual, qk, 0},
                                 sizeof(double), 0, st, flags, 0);
         };
         if (!overlap) {
             if (int rc = class0()) return rc;
+            if (mfill) {
+                MPK_HIP_TRY(hipEventRecord(plan->side.join, plan->side.s));
+                MPK_HIP_TRY(hipStreamWaitEvent(st, plan->side.join, 0));
+            }
             return listed(st);
         }
         // the listed tiles on the side stream, the class-0 pass on the

@@ -36,6 +36,8 @@ struct TileCtx {
     const double4* __restrict__ P4;
     const int32_t* __restrict__ rowind;
     int64_t n;
+    const uint2* __restrict__ inc_ring8;  // compact ring records (plan.ring8)
+    const uint4* __restrict__ col_head;   // their column heads
 };
 
 __device__ __forceinline__ int pos_of(uint2 pw, int r) {

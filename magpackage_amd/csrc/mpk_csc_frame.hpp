@@ -258,33 +258,42 @@ struct ElemState {
     double4 s0;  // Bx, By, Bz, nu
     double2 s1;  // dnu, sigma
 };
+// A state view gathers an element's state as it is stored (Elem / Bnu, kept
+// in that form while the gathers of a ring batch are in flight) and widens it
+// to the FP64 arithmetic's types at the point of use (full / bnu).
 struct StateView {
     using Elem = ElemState;
+    using Bnu = double4;
     using S0 = double4;
     using S1 = double2;
     const double4* __restrict__ s0;
     const double2* __restrict__ s1;
     __device__ Elem load(uint32_t e) const { return Elem{s0[e], s1[e]}; }
-    __device__ double4 load_bnu(uint32_t e) const { return s0[e]; }
+    __device__ Bnu load_bnu(uint32_t e) const { return s0[e]; }
+    __device__ static ElemState full(const Elem& r) { return r; }
+    __device__ static double4 bnu(const Bnu& b) { return b; }
 };
 // FP32-local element state: the same fields rounded to FP32, 24 B per element
-// (half the gather and HBM bytes of the FP64 state), widened to FP64 at the
-// load so the arithmetic after it is the FP64-state code.
+// (half the gather and HBM bytes of the FP64 state), held as FP32 in
+// registers (half the registers per gather in flight) and widened to FP64
+// where the arithmetic reads it, which is then the FP64-state code.
 struct StateView32 {
-    using Elem = ElemState;
+    struct Raw {
+        float4 a;
+        float2 b;
+    };
+    using Elem = Raw;
+    using Bnu = float4;
     using S0 = float4;
     using S1 = float2;
     const float4* __restrict__ s0;
     const float2* __restrict__ s1;
-    __device__ Elem load(uint32_t e) const {
-        const float4 a = s0[e];
-        const float2 b = s1[e];
-        return Elem{make_double4(a.x, a.y, a.z, a.w), make_double2(b.x, b.y)};
+    __device__ Elem load(uint32_t e) const { return Raw{s0[e], s1[e]}; }
+    __device__ Bnu load_bnu(uint32_t e) const { return s0[e]; }
+    __device__ static ElemState full(const Elem& r) {
+        return ElemState{make_double4(r.a.x, r.a.y, r.a.z, r.a.w), make_double2(r.b.x, r.b.y)};
     }
-    __device__ double4 load_bnu(uint32_t e) const {
-        const float4 a = s0[e];
-        return make_double4(a.x, a.y, a.z, a.w);
-    }
+    __device__ static double4 bnu(const Bnu& a) { return make_double4(a.x, a.y, a.z, a.w); }
 };
 
 // Newton Jacobian J = K(nu) + 2 dnu (B.C_r)(B.C_c) and residual R = K(nu) a
@@ -312,8 +321,9 @@ struct FrameJac {
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
-    __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
-                            Acc* acc, Col& cs) const {
+    __device__ void compute(const FrameNodes& nodes, const Elem& raw, uint32_t f,
+                            const FPos& pos, Acc* acc, Col& cs) const {
+        const ElemState el = SV::full(raw);
         const FrameGeom g = frame_geom(nodes, false);
         const Vec3 Bv{el.s0.x, el.s0.y, el.s0.z};
         const double nu = el.s0.w, dnu = el.s1.x;
@@ -387,8 +397,9 @@ struct FrameNewton {
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
-    __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
-                            Acc* acc, Col& cs) const {
+    __device__ void compute(const FrameNodes& nodes, const Elem& raw, uint32_t f,
+                            const FPos& pos, Acc* acc, Col& cs) const {
+        const ElemState el = SV::full(raw);
         const double sigma = el.s1.y;
         const bool has_m = kWithM && fabs(sigma) > 1e-12;
         const FrameGeom g = frame_geom(nodes, has_m);
@@ -439,7 +450,7 @@ struct FrameNewtonLin {
     using Acc = double;
     static constexpr int kRing = 6;  // incidences per gather round (ring driver)
     using Ids = NoIds;
-    using Elem = double4;  // Bx, By, Bz, nu
+    using Elem = typename SV::Bnu;  // Bx, By, Bz, nu as stored
     struct Col {
         double r;
     };
@@ -472,8 +483,9 @@ struct FrameNewtonLin {
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load_bnu(e); }
-    __device__ void compute(const FrameNodes& nodes, const Elem& el, uint32_t f, const FPos& pos,
-                            Acc* acc, Col& cs) const {
+    __device__ void compute(const FrameNodes& nodes, const Elem& raw, uint32_t f,
+                            const FPos& pos, Acc* acc, Col& cs) const {
+        const double4 el = SV::bnu(raw);
         const FrameGeom g = frame_geom(nodes, false);
         const Vec3 Bv{el.x, el.y, el.z};
         const double nu = el.w;
@@ -576,12 +588,17 @@ __global__ void __launch_bounds__(kW) k_csc_frame(TileCtx tc, Op op) {
 // Every index of a tile depends on the tile and the lane only: the loads go
 // out together, one memory latency before the records instead of a chain of
 // dependent ones.
+// kC selects the record format: compact (8-byte records + a 16-byte column
+// head; positions of the shared face and orientation bits carried from one
+// incidence to the next) or the 16-byte ring records.
 struct RingIdx {
-    int cbase, inc0, inc1;
-    int2 lh;
+    int cbase, cnt;  // column's first value index, incidences
+    int lo, hi;      // column edge's endpoints
+    uint4 head;      // compact format: the column head
     int base, top, fbase;
     int32_t cls;
 };
+template <bool kC>
 __device__ __forceinline__ RingIdx ring_load(const TileCtx& tc, const int2* __restrict__ ends,
                                              int64_t tile_id, const int32_t* __restrict__ sel,
                                              int32_t want) {
@@ -590,27 +607,48 @@ __device__ __forceinline__ RingIdx ring_load(const TileCtx& tc, const int2* __re
     const int64_t j = j0 + threadIdx.x;
     const int64_t jc = j < jend ? j : jend - 1;  // a valid column for every lane
     RingIdx ix;
-    ix.cbase = tc.colptr[jc];
-    ix.inc0 = tc.inc_ptr[jc];
-    ix.inc1 = tc.inc_ptr[jc + 1];
-    ix.lh = ends[jc];
     ix.base = tc.colptr[j0];
     ix.top = tc.colptr[jend];
     ix.fbase = tc.frame_base[tile_id];
     ix.cls = sel ? sel[tile_id] : want;
+    if constexpr (kC) {
+        const uint4 h = tc.col_head[jc];
+        ix.head = h;
+        ix.cbase = ix.base + (int)(h.w & 0xffffu);
+        ix.cnt = (int)((h.w >> 16) & 0xffu);
+        ix.lo = (int)(h.x & 0xffffffu);
+        ix.hi = (int)(h.y & 0xffffffu);
+    } else {
+        ix.cbase = tc.colptr[jc];
+        ix.cnt = tc.inc_ptr[jc + 1] - tc.inc_ptr[jc];
+        const int2 lh = ends[jc];
+        ix.lo = lh.x;
+        ix.hi = lh.y;
+        ix.head = make_uint4(0, 0, 0, 0);
+    }
     return ix;
 }
+
+// Compact record fields (layout in mpk_plan.hpp).
+__device__ __forceinline__ uint32_t r8_elem(uint2 r) { return r.x & 0xffffffu; }
+__device__ __forceinline__ int r8_new(uint2 r, int lo) {
+    return lo + ((int)(r.y << 10) >> 10);  // 22-bit two's complement offset from lo
+}
+__device__ __forceinline__ uint32_t r8_pos6(uint32_t v) { return v == 63u ? 0xffu : v; }
 
 // One tile of the ring driver from its loaded indices.  A tile of another
 // class is not branched around (the compiler would hoist that branch above
 // the index loads): it runs with no live lanes and an empty range, so it
 // writes nothing.
-template <class Op, int kB>
+template <class Op, int kB, bool kC>
 __device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned char* smem,
                                          int64_t tile_id, const RingIdx& ix, int32_t want) {
     using Acc = typename Op::Acc;
     using Ids = typename Op::Ids;
     using Elem = typename Op::Elem;
+    using Rec = std::conditional_t<kC, uint2, uint4>;
+    static_assert(!kC || std::is_same<Ids, NoIds>::value,
+                  "compact ring records carry no per-incidence positions for Ids");
     Acc* acc = reinterpret_cast<Acc*>(smem);
     const int lane = threadIdx.x;
     TileRange tr;
@@ -618,8 +656,7 @@ __device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned cha
     tr.j0 = tile_id * kW;
     tr.jend = tr.j0 + kW < tc.n ? tr.j0 + kW : tc.n;
     const int64_t j = tr.j0 + lane;
-    const int cbase = ix.cbase, inc0 = ix.inc0, inc1 = ix.inc1;
-    const int2 lh = ix.lh;
+    const int cbase = ix.cbase;
     const int base = ix.base, top = ix.top, fbase = ix.fbase;
     const bool mine = ix.cls == want;
     const bool live = mine && j < tr.jend;
@@ -630,10 +667,14 @@ __device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned cha
     // record, and to record row 0 when the lane has none (every column of a
     // plan has at least one incidence, so each tile's record block has at
     // least one row of kW records; a lane without records never uses them).
-    const int cnt = live ? inc1 - inc0 : 0;
+    const int cnt = live ? ix.cnt : 0;
     const int klast = cnt > 0 ? cnt - 1 : 0;
-    const uint4* __restrict__ rec = tc.inc_ring + fbase + lane;
-    uint4 r[kB];
+    const Rec* __restrict__ rec;
+    if constexpr (kC)
+        rec = tc.inc_ring8 + fbase + lane;
+    else
+        rec = tc.inc_ring + fbase + lane;
+    Rec r[kB];
 #pragma unroll
     for (int i = 0; i < kB; ++i) r[i] = rec[(i < klast ? i : klast) * kW];
     tile_zero(op, acc, smem, tr);
@@ -641,8 +682,22 @@ __device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned cha
         Acc* col = acc + (cbase - base);
         typename Op::Col cs = op.col_begin();
         if (cnt > 0) {
-            const Vec3 xlo = ld_node(tc.P4, lh.x), xhi = ld_node(tc.P4, lh.y);
-            Vec3 xprev = ld_node(tc.P4, rrec_start(r[0]));
+            const int lo = ix.lo, hi = ix.hi;
+            const Vec3 xlo = ld_node(tc.P4, lo), xhi = ld_node(tc.P4, hi);
+            // compact format: node id of o_prev and the positions carried
+            // along the ring (pos'_0 fixed, pos'_1 / pos'_3 from the last
+            // record's pos'_2 / pos'_4)
+            int oprev = 0;
+            uint32_t p0 = 0, p1 = 0, p3 = 0;
+            if constexpr (kC) {
+                oprev = (int)(ix.head.z & 0xffffffu);
+                p0 = ix.head.x >> 24;
+                p1 = ix.head.y >> 24;
+                p3 = ix.head.z >> 24;
+            } else {
+                oprev = rrec_start(r[0]);
+            }
+            Vec3 xprev = ld_node(tc.P4, oprev);
             for (int k0 = 0; k0 < cnt; k0 += kB) {
                 if (k0 > 0) {
 #pragma unroll
@@ -652,20 +707,54 @@ __device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned cha
                     }
                 }
                 Ids d[kB];
+                int onew[kB];
+                uint32_t elem[kB];
 #pragma unroll
-                for (int i = 0; i < kB; ++i) d[i] = op.ids(tc, cbase, rrec_pos(r[i]));
+                for (int i = 0; i < kB; ++i) {
+                    if constexpr (kC) {
+                        d[i] = Ids{};
+                        onew[i] = r8_new(r[i], lo);
+                        elem[i] = r8_elem(r[i]);
+                    } else {
+                        d[i] = op.ids(tc, cbase, rrec_pos(r[i]));
+                        onew[i] = rrec_new(r[i]);
+                        elem[i] = rrec_elem(r[i]);
+                    }
+                }
                 Vec3 xn[kB];
                 Elem el[kB];
 #pragma unroll
                 for (int i = 0; i < kB; ++i) {
-                    xn[i] = ld_node(tc.P4, rrec_new(r[i]));
-                    el[i] = op.elem(rrec_elem(r[i]), d[i]);
+                    xn[i] = ld_node(tc.P4, onew[i]);
+                    el[i] = op.elem(elem[i], d[i]);
                 }
 #pragma unroll
                 for (int i = 0; i < kB; ++i) {
                     if (k0 + i < cnt) {
-                        op.compute(FrameNodes{{xlo, xhi, xprev, xn[i]}}, el[i], rrec_f(r[i]),
-                                   rrec_pos(r[i]), col, cs);
+                        if constexpr (kC) {
+                            const uint2 q = r[i];
+                            const uint32_t p2 = r8_pos6((q.x >> 24) & 63u);
+                            const uint32_t p4 = r8_pos6((q.y >> 22) & 63u);
+                            const uint32_t p5 = r8_pos6((q.x >> 30) | ((q.y >> 28) << 2));
+                            const int on = onew[i];
+                            const uint32_t f = 1u | (uint32_t)(lo < oprev) << 1 |
+                                               (uint32_t)(lo < on) << 2 |
+                                               (uint32_t)(hi < oprev) << 3 |
+                                               (uint32_t)(hi < on) << 4 |
+                                               (uint32_t)(oprev < on) << 5;
+                            op.compute(FrameNodes{{xlo, xhi, xprev, xn[i]}}, el[i], f,
+                                       FPos{p0 | p1 << 8 | p2 << 16 | p3 << 24, p4 | p5 << 8},
+                                       col, cs);
+                            p1 = p2;
+                            p3 = p4;
+                            oprev = on;
+                        } else {
+                            // bit 0 (lo -> hi) is 1 in every ring record; set
+                            // as a constant, both formats compile the same
+                            // arithmetic (and round identically)
+                            op.compute(FrameNodes{{xlo, xhi, xprev, xn[i]}}, el[i],
+                                       rrec_f(r[i]) | 1u, rrec_pos(r[i]), col, cs);
+                        }
                         xprev = xn[i];
                     }
                 }
@@ -676,29 +765,29 @@ __device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned cha
     tile_end(op, tr, acc);
 }
 
-template <class Op, int kB>
+template <class Op, int kB, bool kC>
 __device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restrict__ ends, Op& op,
                                           unsigned char* smem, int64_t tile_id,
                                           const int32_t* __restrict__ sel = nullptr,
                                           int32_t want = 0) {
-    const RingIdx ix = ring_load(tc, ends, tile_id, sel, want);
-    ring_run<Op, kB>(tc, op, smem, tile_id, ix, want);
+    const RingIdx ix = ring_load<kC>(tc, ends, tile_id, sel, want);
+    ring_run<Op, kB, kC>(tc, op, smem, tile_id, ix, want);
 }
 
 // sel (optional): per-tile class flags; a block whose tile's flag differs
 // from `want` returns at once (the tiles of a plan split over two launches).
-template <class Op, int kB>
+template <class Op, int kB, bool kC>
 __global__ void __launch_bounds__(kW) k_csc_ring_batch(TileCtx tc, const int2* __restrict__ ends,
                                                        Op op, const int32_t* __restrict__ sel,
                                                        int32_t want) {
     extern __shared__ __align__(16) unsigned char smem[];
-    ring_tile<Op, kB>(tc, ends, op, smem, xcd_tile(blockIdx.x, gridDim.x), sel, want);
+    ring_tile<Op, kB, kC>(tc, ends, op, smem, xcd_tile(blockIdx.x, gridDim.x), sel, want);
 }
 
 // Tiles from a device list (persistent grid): block b runs on XCD b % 8; the
 // blocks of XCD x walk the contiguous list range [x c / 8, (x+1) c / 8),
 // c = *count, so neighbouring tiles of the list share that XCD's L2.
-template <class Op, int kB>
+template <class Op, int kB, bool kC>
 __global__ void __launch_bounds__(kW) k_csc_ring_list(TileCtx tc, const int2* __restrict__ ends,
                                                       Op op, const int32_t* __restrict__ list,
                                                       const int32_t* __restrict__ count) {
@@ -709,7 +798,7 @@ __global__ void __launch_bounds__(kW) k_csc_ring_list(TileCtx tc, const int2* __
     const int64_t nbx = ((int64_t)gridDim.x - x + kXcd - 1) / kXcd;  // blocks on XCD x
     const int64_t lo = c * x / kXcd, hi = c * (x + 1) / kXcd;
     for (int64_t p = lo + blockIdx.x / kXcd; p < hi; p += nbx) {
-        ring_tile<Op, kB>(tc, ends, op, smem, list[p]);
+        ring_tile<Op, kB, kC>(tc, ends, op, smem, list[p]);
         __syncthreads();
     }
 }

@@ -67,6 +67,27 @@ struct mpk_plan {
     uint4* inc_ring = nullptr;
     int2* col_ends = nullptr;  // [n] (lo, hi) global node ids of each column edge
     mpk::DevBuf buf_incring, buf_colends;
+
+    // Compact ring records (8 B per incidence), built from the ring records
+    // when every field fits; the 16-byte ring records are then released.
+    // Consecutive elements of a ring share the face (lo, hi, o_prev), so the
+    // positions of the frame edges (lo, o_prev) and (hi, o_prev) are the
+    // previous record's (lo, o_new) and (hi, o_new); the orientation bits
+    // follow from the four node ids.  Per incidence (uint2, same k-major
+    // tile layout and frame_base):
+    //   x = element (24 bits) | pos'_2 << 24 (6 bits) | (pos'_5 & 3) << 30
+    //   y = (o_new - lo) as 22-bit two's complement | pos'_4 << 22 (6 bits)
+    //       | (pos'_5 >> 2) << 28
+    // (6-bit positions, 63 = inactive row).  Per column (uint4):
+    //   x = lo | pos'_0 << 24, y = hi | pos'_1 << 24, z = o_start | pos'_3 << 24
+    //   (first record's pos'_1 / pos'_3, 8 bits, 0xff = inactive),
+    //   w = colptr[j] - colptr[tile start] (16 bits) | incidences << 16 (8 bits).
+    // Needs nodes < 2^24, |o_new - lo| < 2^21, column nnz <= 63, <= 255
+    // incidences per column and tiles of <= 65535 entries.
+    bool ring8 = false;
+    uint2* inc_ring8 = nullptr;
+    uint4* col_head = nullptr;
+    mpk::DevBuf buf_incring8, buf_colhead;
     // Node coordinates padded to 32 bytes (x, y, z, 0), refreshed from the
     // caller's P by mpk_plan_set_coords and by every frame-kernel call given P.
     double4* P4 = nullptr;

@@ -10,6 +10,7 @@
 //   mpk_sparse_coo_to_csc  src/Assembler.m:37-38 (I>0 & J>0 filter + sparse())
 //   mpk_plan_create        the same sparse() pattern, built once, plus the
 //                          per-column incidence lists used by the CSC kernels
+#include <cstdlib>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -478,6 +479,71 @@ __global__ void k_plan_ring(const int32_t* __restrict__ inc_ptr,
     }
 }
 
+// Compact ring records and column heads (layout in mpk_plan.hpp) from the
+// 16-byte ring records, one thread per column.  The conversion checks what
+// the compact form derives instead of storing: the orientation bits computed
+// from the four node ids must equal the stored ones, every record's
+// (lo, o_prev) / (hi, o_prev) positions must be the previous record's
+// (lo, o_new) / (hi, o_new) ones, and pos'_0 must be the same for the whole
+// column.  Any mismatch or field out of range sets err (the plan then keeps
+// the 16-byte records).
+__global__ void k_plan_ring8(const int32_t* __restrict__ inc_ptr,
+                             const int32_t* __restrict__ colptr,
+                             const int32_t* __restrict__ frame_base, int64_t n,
+                             const uint4* __restrict__ rec16, const int2* __restrict__ ends,
+                             uint2* __restrict__ rec8, uint4* __restrict__ head,
+                             int* __restrict__ err) {
+    const int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (j >= n) return;
+    const int d = inc_ptr[j + 1] - inc_ptr[j];
+    const int64_t j0 = (j / kTileCols) * kTileCols;
+    const int cofs = colptr[j] - colptr[j0];
+    const int2 lh = ends[j];
+    if (d > 255 || cofs > 0xffff || lh.x < 0 || lh.y >= (1 << 24)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int64_t base = frame_base[j / kTileCols] + (j % kTileCols);
+    auto pos_of = [](uint4 r, int rp) -> uint32_t {
+        // ring record: pos'_0 in y's top byte, pos'_1..pos'_4 in z, pos'_5 in w
+        return rp == 0 ? (r.y >> 24) : rp < 5 ? ((r.z >> (8 * (rp - 1))) & 0xffu) : (r.w & 0xffu);
+    };
+    auto six = [&](uint32_t p) -> uint32_t {
+        if (p == 0xffu) return 63u;
+        if (p > 62u) atomicOr(err, 1);
+        return p & 63u;
+    };
+    uint32_t p0 = 0xffu, p1 = 0xffu, p3 = 0xffu, q2 = 0, q4 = 0;
+    int ostart = 0, oprev = 0;
+    for (int k = 0; k < d; ++k) {
+        const uint4 r = rec16[base + (int64_t)k * kTileCols];
+        const uint32_t e = r.x & 0xffffffu;
+        const int onew = (int)(r.y & 0xffffffu);
+        if (k == 0) {
+            p0 = pos_of(r, 0);
+            p1 = pos_of(r, 1);
+            p3 = pos_of(r, 3);
+            ostart = oprev = (int)(r.w >> 8);
+        } else if (pos_of(r, 0) != p0 || pos_of(r, 1) != q2 || pos_of(r, 3) != q4) {
+            atomicOr(err, 1);
+        }
+        const uint32_t f = 1u | (uint32_t)(lh.x < oprev) << 1 | (uint32_t)(lh.x < onew) << 2 |
+                           (uint32_t)(lh.y < oprev) << 3 | (uint32_t)(lh.y < onew) << 4 |
+                           (uint32_t)(oprev < onew) << 5;
+        const int dn = onew - lh.x;
+        if (f != ((r.x >> 24) & 0x3fu) || dn < -(1 << 21) || dn >= (1 << 21)) atomicOr(err, 1);
+        const uint32_t s2 = six(pos_of(r, 2)), s4 = six(pos_of(r, 4)), s5 = six(pos_of(r, 5));
+        rec8[base + (int64_t)k * kTileCols] =
+            make_uint2(e | s2 << 24 | (s5 & 3u) << 30,
+                       ((uint32_t)dn & 0x3fffffu) | s4 << 22 | (s5 >> 2) << 28);
+        q2 = pos_of(r, 2);
+        q4 = pos_of(r, 4);
+        oprev = onew;
+    }
+    head[j] = make_uint4((uint32_t)lh.x | p0 << 24, (uint32_t)lh.y | p1 << 24,
+                         (uint32_t)ostart | p3 << 24, (uint32_t)cofs | (uint32_t)d << 16);
+}
+
 __global__ void k_tile_max(const int32_t* __restrict__ colptr, int64_t n,
                            int* __restrict__ maxv) {
     const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
@@ -882,6 +948,36 @@ int mpk_plan_create(const int32_t* T, const int32_t* cell_dofs, int64_t ne, int6
                 plan->buf_incring.release();
                 plan->buf_colends.release();
             }
+            // compact ring records, unless MPK_RING_FORMAT=16 keeps the
+            // 16-byte ones (for comparisons and timing)
+            const char* fmt = std::getenv("MPK_RING_FORMAT");
+            if (plan->ring && !(fmt && std::strcmp(fmt, "16") == 0)) {
+                if (plan->buf_incring8.alloc((size_t)nslots * 8) != hipSuccess ||
+                    plan->buf_colhead.alloc((size_t)ndofs * 16) != hipSuccess) {
+                    set_error("hipMalloc failed for compact ring records");
+                    return fail(MPK_ERR_HIP);
+                }
+                (void)hipMemsetAsync(err.p, 0, sizeof(int), st);
+                hipLaunchKernelGGL(k_plan_ring8, dim3(grid_for(ndofs, kB)), dim3(kB), 0, st,
+                                   plan->inc_ptr, plan->colptr, plan->frame_base, ndofs,
+                                   plan->inc_ring, plan->col_ends,
+                                   plan->buf_incring8.as<uint2>(), plan->buf_colhead.as<uint4>(),
+                                   err.as<int>());
+                int h8 = 0;
+                if (int rc = check_err(err.as<int>(), st, &h8)) return fail(rc);
+                plan->ring8 = (h8 == 0);
+                if (plan->ring8) {
+                    plan->inc_ring8 = plan->buf_incring8.as<uint2>();
+                    plan->col_head = plan->buf_colhead.as<uint4>();
+                    plan->inc_ring = nullptr;
+                    plan->col_ends = nullptr;
+                    plan->buf_incring.release();
+                    plan->buf_colends.release();
+                } else {
+                    plan->buf_incring8.release();
+                    plan->buf_colhead.release();
+                }
+            }
         }
     }
     if (plan->buf_P4.alloc((size_t)(nn > 0 ? nn : 1) * 32) != hipSuccess) {
@@ -917,7 +1013,7 @@ int mpk_plan_info(const mpk_plan* plan, int64_t* nnz, int64_t* ncols, int64_t* n
 int mpk_plan_ring_order(const mpk_plan* plan, int* ring) {
     clear_error();
     MPK_REQUIRE(plan && ring, MPK_ERR_ARGS, "NULL argument");
-    *ring = plan->ring ? 1 : 0;
+    *ring = plan->ring ? (plan->ring8 ? 2 : 1) : 0;
     return MPK_OK;
 }
 
