@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X assembly engine (one GPU).
 
-Default workload: frequency-domain eddy-current operator K + 1j*omega*M_sigma
-as a complex128 CSC matrix on a 128^3-cube jittered Kuhn box (12,582,912
-tets, 14,827,904 edge DOFs), a conducting plate (centroid z in
-[0.45, 0.55], sigma = 3.5e7 S/m), mu_r = 1 everywhere, 50 Hz.  One step =
+Default line (`python bench.py`): frequency-domain eddy-current operator
+K + 1j*omega*M_sigma as a complex128 CSC matrix on a 128^3-cube jittered Kuhn
+box (12,582,912 tets, 14,827,904 edge DOFs), a conducting plate (centroid z
+in [0.45, 0.55], sigma = 3.5e7 S/m), mu_r = 1 everywhere, 50 Hz.  One step =
 loading the caller's coordinates into the plan + one assembly launch over
 the whole mesh, inputs resident in HBM, the CSC pattern plan built once
 before timing (its cost is reported separately).  ``value`` is the step
 rate; the roofline figure divides the step's algorithmic bytes by the
-assembly kernel's own duration (HIP events around that launch).
+assembly kernel's own duration (HIP events around that launch).  The same
+line carries, under extra keys:
+  fp32_local          the operator's FP32-local mode (time, error vs FP64);
+  newton_set          the fused K(nu(B)) + M_sigma + J + R set on the
+                      100x100x67 box (4.02M tets), FP32-local, with its own
+                      roofline, traffic and CPU baseline;
+  nonlinear_step_time the E-core transient (983,040 tets), 200 backward-Euler
+                      steps of 1 ms: plain Newton against the learned warm
+                      start trained on steps 1-40, over steps 41-200.
 
 Prints ONE JSON line (rank 0).  The timed region is exactly --steps launches
 between two device synchronisations; the kernel duration is measured with
@@ -124,8 +132,9 @@ def cpu_baseline_newton(sample: int, reps: int):
     }
 
 
-def run_newton(args):
-    """K(nu(B)) + M_sigma + J + R in one fused pass, FP32-local by default."""
+def run_newton(args, emit=True):
+    """K(nu(B)) + M_sigma + J + R in one fused pass, FP32-local by default.
+    Returns the result line (printed when ``emit``)."""
     import torch
     from magpackage_amd import _lib as L
     from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
@@ -181,6 +190,7 @@ def run_newton(args):
             ev_b.record(stream)
 
     ms_per, ms_kernel = timed_steps(step, stream, args.steps, args.warmup)
+    torch.cuda.synchronize()
     ne, nn, nnz = km.num_elems, km.num_nodes, plan.nnz
     value = ne / (ms_per * 1e-3)
     alg = algorithmic_bytes_newton(nn, ne, n, nnz)
@@ -188,6 +198,18 @@ def run_newton(args):
     log(f"[bench] {args.steps} steps: {ms_per:.4f} ms/step (events), "
         f"{'K+J+R pass + M pass' if args.split_m else 'fused K+M+J+R'} "
         f"{ms_kernel:.4f} ms, {value:.4g} tets/s, {achieved:.0f} GB/s algorithmic")
+    # the same set in FP64: its kernel time and the FP32-local error per output
+    err = None
+    if args.precision == "fp32_local":
+        out32 = [t.clone() for t in (kv, mv, jv, rv)]
+        prec = L.PRECISIONS["fp64"]
+        _, ms64 = timed_steps(step, stream, args.steps, args.warmup)
+        err = {"fp64_kernel_ms": ms64}
+        for name, a32, a64 in zip("KMJR", out32, (kv, mv, jv, rv)):
+            err[f"max_rel_err_{name}"] = float((a32 - a64).abs().max() / a64.abs().max())
+        log(f"[bench] fp64 set {ms64:.4f} ms; fp32-local errors " +
+            " ".join(f"{k[-1]} {v:.1e}" for k, v in err.items() if k.startswith("max")))
+        del out32
     if args.calib:
         calib_copy(dev)
     cpu = None
@@ -196,7 +218,7 @@ def run_newton(args):
         cpu = cpu_baseline_newton(args.cpu_sample, args.cpu_reps)
         log(f"[bench] cpu: {cpu['value']:.4g} tets/s")
     workload = "kuhn100x100x67_newton_kmjr"
-    print(json.dumps({
+    line = {
         "metric": METRIC, "value": value, "unit": "tets/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
@@ -210,7 +232,11 @@ def run_newton(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(workload),
                      "algorithmic_bytes": alg, "kernel_ms": ms_kernel},
         "cpu_baseline": cpu,
-    }), flush=True)
+        "fp32_local_vs_fp64": err,
+    }
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def run_ecore(args, emit=True):
@@ -530,7 +556,9 @@ def main():
     ap.add_argument("--warm-eta", type=float, default=1e-2,
                     help="ecore: relative tolerance of the learned warm start's solve")
     ap.add_argument("--no-nonlinear", action="store_true",
-                    help="freq128: skip the short learned-vs-plain nonlinear transient")
+                    help="freq128: skip the 200-step learned-vs-plain nonlinear transient")
+    ap.add_argument("--no-newton", action="store_true",
+                    help="freq128: skip the fused Newton set on the 100x100x67 box")
     ap.add_argument("--raw-source", action="store_true",
                     help="ecore: use the per-element winding current as is (not projected "
                          "onto divergence-free loads)")
@@ -623,6 +651,7 @@ def main():
             ev_b.record(stream)
 
     ms_per, ms_kernel = timed_steps(step, stream, args.steps, args.warmup)
+    torch.cuda.synchronize()
     ne, nn, nnz = km.num_elems, km.num_nodes, plan.nnz
     value = ne / (ms_per * 1e-3)
     alg = algorithmic_bytes_freq(nn, ne, nnz)
@@ -630,6 +659,24 @@ def main():
     log(f"[bench] {args.steps} steps: {ms_per:.4f} ms/step (events), assembly kernel "
         f"{ms_kernel:.4f} ms, {value:.4g} tets/s, "
         f"{achieved:.0f} GB/s algorithmic")
+    # the FP32-local mode of the same operator, timed the same way, with its
+    # error against the FP64 values just computed
+    fp32 = None
+    if args.precision == "fp64":
+        ref = vals.clone()
+        prec = L.PRECISIONS["fp32_local"]
+        ms32_step, ms32 = timed_steps(step, stream, args.steps, args.warmup)
+        torch.cuda.synchronize()
+        v32, v64 = vals.view(-1, 2), ref.view(-1, 2)
+        errs = [float((v32[:, k] - v64[:, k]).abs().max() / v64[:, k].abs().max())
+                for k in (0, 1)]
+        ach32 = alg / (ms32 * 1e-3) / 1e9
+        fp32 = {"ms_per_step": ms32_step, "kernel_ms": ms32, "value": ne / (ms32_step * 1e-3),
+                "achieved_gbs": ach32, "frac": ach32 / HBM_PEAK_GBS,
+                "max_rel_err_re": errs[0], "max_rel_err_im": errs[1]}
+        log(f"[bench] fp32-local: kernel {ms32:.4f} ms ({ach32:.0f} GB/s), max rel err "
+            f"Re {errs[0]:.1e} Im {errs[1]:.1e}")
+        del ref, v32, v64
 
     if args.calib:
         calib_copy(dev)
@@ -640,13 +687,29 @@ def main():
         cpu = cpu_baseline(args.cpu_sample, args.cpu_reps)
         log(f"[bench] cpu: {cpu['value']:.4g} tets/s")
 
+    ndofs = int(dh.NumGlobalDofs)
+    del vals, nu_d, sg_d, plan, asm, dh, mesh, km
+    torch.cuda.empty_cache()
+
+    newton = None
+    if not args.no_newton:
+        # the fused K + M + J + R Newton set on the 100x100x67 box (FP32-local,
+        # with its FP64 time and error), roofline, traffic and CPU baseline
+        na = argparse.Namespace(**vars(args))
+        na.precision, na.calib = "fp32_local", False
+        na.cpu_sample = 64
+        newton = run_newton(na, emit=False)
+        for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline"):
+            newton.pop(k, None)
+        torch.cuda.empty_cache()
+
     nonlinear = None
     if not args.no_nonlinear:
-        # learned-vs-plain nonlinear transient, short form (the full 200-step
-        # run is --workload ecore)
-        del vals, nu_d, sg_d
+        # learned-vs-plain nonlinear transient: 200 backward-Euler steps of
+        # 1 ms (ten 50 Hz periods), the network trained on steps 1-40, plain
+        # Newton and the learned warm start compared over steps 41-200
         ea = argparse.Namespace(**vars(args))
-        ea.steps, ea.train_steps, ea.dt = 60, 40, 1e-3
+        ea.steps, ea.train_steps, ea.dt = 200, 40, 1e-3
         ea.only, ea.no_extrap_run, ea.verbose_newton = None, True, False
         r = run_ecore(ea, emit=False)
         nonlinear = {k: r[k] for k in ("eval_steps", "speedup_learned_vs_plain", "config")}
@@ -673,7 +736,7 @@ def main():
         "config": {
             "workload": workload,
             "operator": "K + 1j*omega*M_sigma, complex128 CSC",
-            "tets": ne, "nodes": nn, "edge_dofs": int(dh.NumGlobalDofs), "nnz": int(nnz),
+            "tets": ne, "nodes": nn, "edge_dofs": ndofs, "nnz": int(nnz),
             "precision": args.precision, "frequency_hz": 50.0, "plate_sigma": 3.5e7,
             "plan_build_ms": round(1e3 * (t3 - t2), 1),
             "edges_dofs_ms": round(1e3 * (t2 - t1), 1),
@@ -689,6 +752,8 @@ def main():
             "kernel_ms": ms_kernel,
         },
         "cpu_baseline": cpu,
+        "fp32_local": fp32,
+        "newton_set": newton,
         "nonlinear_step_time": nonlinear,
     }
     print(json.dumps(out), flush=True)
