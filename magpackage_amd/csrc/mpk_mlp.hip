@@ -1,13 +1,23 @@
 // Learned reluctivity predictor: a 16 -> 64 -> 64 -> 64 -> 1 tanh MLP whose
 // hidden layers run on the gfx950 matrix cores.
 //
-// Each wave64 owns 16-row chunks.  Layer products are 16x16x16 tiles of
-// v_mfma_f32_16x16x16bf16_1k (BF16 operands, FP32 accumulation):
-//   A (16 x 16):  lane l holds A[l%16][4*(l/16) + i], i = 0..3
-//   B (16 x 16):  lane l holds B[4*(l/16) + i][l%16]
-//   D (16 x 16):  lane l holds D[4*(l/16) + i][l%16]
+// Each wave64 owns 16-row chunks.  Layer products are tiles of gfx950's BF16
+// MFMA (BF16 operands, FP32 accumulation):
+//   16x16x32 (v_mfma_f32_16x16x32_bf16, the 64-wide layers 2 and 3 and the
+//   backward dH products): lane l holds A[l%16][8*(l/16) + i] and
+//   B[8*(l/16) + i][l%16], i = 0..7;
+//   16x16x16 (v_mfma_f32_16x16x16bf16_1k, the 16-wide layer 1 and the weight
+//   gradients, whose reduction runs over the chunk's 16 rows): lane l holds
+//   A[l%16][4*(l/16) + i], B[4*(l/16) + i][l%16], i = 0..3;
+//   D (16 x 16) of both: lane l holds D[4*(l/16) + i][l%16].
+// The instruction pairs A's and B's register elements by (lane group, i), so
+// loading both operands with the same k numbering gives the product.
 // Weights (BF16) and the chunk's activations (BF16) sit in LDS; D tiles go
 // through LDS to become the next layer's A operands.
+//
+// Activations: tanh by a rational form (tanh_fast below), about 15 VALU
+// instructions against ~40 for the library tanhf; BF16 rounding by the
+// hardware conversion (round to nearest even).
 //
 // Training step (mpk_mlp_loss_grad): forward, MSE loss, backward through the
 // three tanh layers.  Weight gradients are MFMA products with the chunk's 16
@@ -27,6 +37,7 @@ namespace mpk {
 namespace {
 
 typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = MPK_MLP_IN;
@@ -44,12 +55,8 @@ constexpr int bW1 = 0, bW2 = 1024, bW3 = 5120;
 static_assert(ob4 + 1 == kNP, "parameter layout");
 static_assert(bW3 + kH * kH == MPK_MLP_NBF16, "bf16 layout");
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ bf4 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
@@ -75,12 +82,55 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// 8 consecutive BF16 of a row-major LDS tile, one 16-byte read (idx is a
+// multiple of 8 at every call and the tiles are 16-byte aligned)
+__device__ __forceinline__ bf8 ld_row8(const uint16_t* t, int idx) {
+    bf8 v;
+    __builtin_memcpy(&v, t + idx, sizeof(v));
+    return v;
+}
 // 4 BF16 of one column, rows r..r+3, row stride ld
 __device__ __forceinline__ bf4 ld_col4(const uint16_t* t, int idx, int ld) {
     return pack4(t[idx], t[idx + ld], t[idx + 2 * ld], t[idx + 3 * ld]);
 }
 __device__ __forceinline__ f4 mfma(bf4 a, bf4 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+// 8 BF16 of one column, rows r..r+7, row stride ld
+__device__ __forceinline__ bf8 ld_col8(const uint16_t* t, int idx, int ld) {
+    uint16_t u[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) u[i] = t[idx + i * ld];
+    bf8 v;
+    __builtin_memcpy(&v, u, sizeof(v));
+    return v;
+}
+
+// tanh(x) by the rational minimax form of Eigen's generic_fast_tanh_float
+// (the form XLA uses too): x P(x^2) / Q(x^2), P of degree 6 and Q of degree 3
+// in x^2, on x clamped to +-7.90531110763549805.  |error| <= 2.6e-7 in exact
+// arithmetic and about 4e-7 in FP32, far below the BF16 rounding (2^-9
+// relative) the activation takes next.  oracle/mlp_oracle.py evaluates the
+// same rational.
+__device__ __forceinline__ float tanh_fast(float x) {
+    constexpr float kClamp = 7.90531110763549805f;
+    x = fminf(fmaxf(x, -kClamp), kClamp);
+    const float x2 = x * x;
+    float p = -2.76076847742355e-16f;
+    p = fmaf(p, x2, 2.00018790482477e-13f);
+    p = fmaf(p, x2, -8.60467152213735e-11f);
+    p = fmaf(p, x2, 5.12229709037114e-08f);
+    p = fmaf(p, x2, 1.48572235717979e-05f);
+    p = fmaf(p, x2, 6.37261928875436e-04f);
+    p = fmaf(p, x2, 4.89352455891786e-03f);
+    float q = 1.19825839466702e-06f;
+    q = fmaf(q, x2, 1.18534705686654e-04f);
+    q = fmaf(q, x2, 2.26843463243900e-03f);
+    q = fmaf(q, x2, 4.89352518554385e-03f);
+    return x * p * __builtin_amdgcn_rcpf(q);
 }
 __device__ __forceinline__ f4 zero4() { return f4{0.f, 0.f, 0.f, 0.f}; }
 
@@ -134,7 +184,7 @@ __device__ void forward_chunk(Smem& s, int w, const float* __restrict__ X, int64
                       zero4());
         const int o = ob * 16 + c;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h1[(4 * g + i) * kH + o] = f2bf(tanhf(acc[i] + s.b1[o]));
+        for (int i = 0; i < 4; ++i) h1[(4 * g + i) * kH + o] = f2bf(tanh_fast(acc[i] + s.b1[o]));
     }
     wave_sync();
     // layer 2
@@ -142,12 +192,12 @@ __device__ void forward_chunk(Smem& s, int w, const float* __restrict__ X, int64
     for (int ob = 0; ob < 4; ++ob) {
         f4 acc = zero4();
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb)
-            acc = mfma(ld_row4(h1, c * kH + kb * 16 + 4 * g),
-                       ld_row4(s.w2, (ob * 16 + c) * kH + kb * 16 + 4 * g), acc);
+        for (int kp = 0; kp < 2; ++kp)
+            acc = mfma32(ld_row8(h1, c * kH + kp * 32 + 8 * g),
+                         ld_row8(s.w2, (ob * 16 + c) * kH + kp * 32 + 8 * g), acc);
         const int o = ob * 16 + c;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h2[(4 * g + i) * kH + o] = f2bf(tanhf(acc[i] + s.b2[o]));
+        for (int i = 0; i < 4; ++i) h2[(4 * g + i) * kH + o] = f2bf(tanh_fast(acc[i] + s.b2[o]));
     }
     wave_sync();
     // layer 3 and the linear output
@@ -156,13 +206,13 @@ __device__ void forward_chunk(Smem& s, int w, const float* __restrict__ X, int64
     for (int ob = 0; ob < 4; ++ob) {
         f4 acc = zero4();
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb)
-            acc = mfma(ld_row4(h2, c * kH + kb * 16 + 4 * g),
-                       ld_row4(s.w3, (ob * 16 + c) * kH + kb * 16 + 4 * g), acc);
+        for (int kp = 0; kp < 2; ++kp)
+            acc = mfma32(ld_row8(h2, c * kH + kp * 32 + 8 * g),
+                         ld_row8(s.w3, (ob * 16 + c) * kH + kp * 32 + 8 * g), acc);
         const int o = ob * 16 + c;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const uint16_t hb = f2bf(tanhf(acc[i] + s.b3[o]));
+            const uint16_t hb = f2bf(tanh_fast(acc[i] + s.b3[o]));
             h3[(4 * g + i) * kH + o] = hb;
             hb3[ob][i] = bf2f(hb);
             part[i] += hb3[ob][i] * s.w4[o];
@@ -278,9 +328,9 @@ k_mlp_grad(const float* __restrict__ p, const uint16_t* __restrict__ wb,
         for (int kb = 0; kb < 4; ++kb) {
             f4 acc = zero4();
 #pragma unroll
-            for (int ob = 0; ob < 4; ++ob)
-                acc = mfma(ld_row4(dz, c * kH + ob * 16 + 4 * g),
-                           ld_col4(s.w3, (ob * 16 + 4 * g) * kH + kb * 16 + c, kH), acc);
+            for (int kp = 0; kp < 2; ++kp)
+                acc = mfma32(ld_row8(dz, c * kH + kp * 32 + 8 * g),
+                             ld_col8(s.w3, (kp * 32 + 8 * g) * kH + kb * 16 + c, kH), acc);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float hb = bf2f(h2[(4 * g + i) * kH + kb * 16 + c]);
@@ -307,9 +357,9 @@ k_mlp_grad(const float* __restrict__ p, const uint16_t* __restrict__ wb,
         for (int kb = 0; kb < 4; ++kb) {
             f4 acc = zero4();
 #pragma unroll
-            for (int ob = 0; ob < 4; ++ob)
-                acc = mfma(ld_row4(dz, c * kH + ob * 16 + 4 * g),
-                           ld_col4(s.w2, (ob * 16 + 4 * g) * kH + kb * 16 + c, kH), acc);
+            for (int kp = 0; kp < 2; ++kp)
+                acc = mfma32(ld_row8(dz, c * kH + kp * 32 + 8 * g),
+                             ld_col8(s.w2, (kp * 32 + 8 * g) * kH + kb * 16 + c, kH), acc);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float hb = bf2f(h1[(4 * g + i) * kH + kb * 16 + c]);
