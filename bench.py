@@ -16,8 +16,10 @@ line carries, under extra keys:
                       100x100x67 box (4.02M tets), FP32-local, with its own
                       roofline, traffic and CPU baseline;
   nonlinear_step_time the E-core transient (983,040 tets), 200 backward-Euler
-                      steps of 1 ms: plain Newton against the learned warm
-                      start trained on steps 1-40, over steps 41-200.
+                      steps of 1 ms: plain Newton, Newton from the extrapolated
+                      start and the learned warm start trained on steps 1-40,
+                      over steps 41-200 (inexact Newton; --workload ecore
+                      --newton-linear-tol 1e-10 gives the exact-solve regime).
 
 Prints ONE JSON line (rank 0).  The timed region is exactly --steps launches
 between two device synchronisations; the kernel duration is measured with
@@ -280,6 +282,7 @@ def run_ecore(args, emit=True):
     def run(kind: str):
         solver = TransientSolver(asm)
         solver.Verbose = 2 if args.verbose_newton else True
+        solver.NewtonLinearTolerance = args.newton_linear_tol
         hook, on_step, pred = None, None, None
         if kind == "extrapolated":
             solver.InitialGuess = "extrapolated"
@@ -327,8 +330,18 @@ def run_ecore(args, emit=True):
                 f"{r['newton']:.2f} Newton, {r['pcg']:.0f} PCG iterations")
     log(f"[ecore] learned vs plain: {speed:.2f}x")
     res["speedup_learned_vs_plain"] = speed
+    if "extrapolated" in res:
+        # what the network adds over the extrapolated start it corrects
+        res["speedup_learned_vs_extrapolated"] = (res["extrapolated"]["ms_per_step"]
+                                                  / learn["ms_per_step"])
+        res["speedup_extrapolated_vs_plain"] = (plain["ms_per_step"]
+                                                / res["extrapolated"]["ms_per_step"])
+        log(f"[ecore] learned vs extrapolated: {res['speedup_learned_vs_extrapolated']:.2f}x")
     res["config"] = {"workload": "ecore_transient_learned_vs_plain", "tets": ne, "edge_dofs": n,
                      "dt": dt, "steps": args.steps, "train_steps": k0, "rel_tol": 1e-6,
+                     "newton_linear_solve": ("eisenstat-walker forcing"
+                                             if args.newton_linear_tol is None else
+                                             f"relative residual {args.newton_linear_tol:g}"),
                      "source": "raw" if args.raw_source else "divergence-free projection"}
     if emit:
         print(json.dumps({
@@ -555,6 +568,10 @@ def main():
     ap.add_argument("--batch-freqs", type=int, default=32, help="batch: number of frequencies")
     ap.add_argument("--warm-eta", type=float, default=1e-2,
                     help="ecore: relative tolerance of the learned warm start's solve")
+    ap.add_argument("--newton-linear-tol", type=float, default=None,
+                    help="ecore: solve every Newton step's linear system to this relative "
+                         "residual (1e-10: the reference's exact direct-solve regime) instead "
+                         "of the Eisenstat-Walker forcing")
     ap.add_argument("--no-nonlinear", action="store_true",
                     help="freq128: skip the 200-step learned-vs-plain nonlinear transient")
     ap.add_argument("--no-newton", action="store_true",
@@ -706,14 +723,18 @@ def main():
     nonlinear = None
     if not args.no_nonlinear:
         # learned-vs-plain nonlinear transient: 200 backward-Euler steps of
-        # 1 ms (ten 50 Hz periods), the network trained on steps 1-40, plain
-        # Newton and the learned warm start compared over steps 41-200
+        # 1 ms (ten 50 Hz periods), the network trained on steps 1-40; plain
+        # Newton, Newton from the extrapolated start and the learned warm
+        # start compared over steps 41-200
         ea = argparse.Namespace(**vars(args))
         ea.steps, ea.train_steps, ea.dt = 200, 40, 1e-3
-        ea.only, ea.no_extrap_run, ea.verbose_newton = None, True, False
+        ea.only, ea.no_extrap_run, ea.verbose_newton = None, False, False
+        ea.newton_linear_tol = None
         r = run_ecore(ea, emit=False)
-        nonlinear = {k: r[k] for k in ("eval_steps", "speedup_learned_vs_plain", "config")}
-        for kind in ("plain", "learned"):
+        nonlinear = {k: r[k] for k in ("eval_steps", "speedup_learned_vs_plain",
+                                       "speedup_learned_vs_extrapolated",
+                                       "speedup_extrapolated_vs_plain", "config")}
+        for kind in ("plain", "extrapolated", "learned"):
             nonlinear[kind] = {k: r[kind][k] for k in ("ms_per_step", "newton", "pcg", "converged")}
         nonlinear["learned"]["train"] = r["learned"]["train"]
         nonlinear["learned"]["warm_used"] = r["learned"]["warm_used"]

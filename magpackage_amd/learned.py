@@ -3,14 +3,18 @@
 No reference counterpart (MagPackage has no learned component).  For every
 element of a nonlinear (B-H) region, a small MLP (include/mpk_mlp.h: 16 ->
 64 -> 64 -> 64 -> 1, tanh, BF16 weights on the matrix cores, FP32
-accumulation) maps
+accumulation) maps the element's recent flux history
 
-    (B_prev, B_extrap = 2 B_prev - B_prev2, B_prev2, |B_prev|^2, |B_extrap|^2,
-     drive(t_next), region)
+    B1, B2, B3    converged B of the last three steps (B_{n-1}, B_{n-2}, B_{n-3})
+    Be  = 2 B1 - B2            (linear extrapolation to step n)
+    dB  = B1 - (2 B2 - B3)     (how far the last step moved off its own
+                                extrapolation)
+    Bq  = 3 B1 - 3 B2 + B3     (quadratic extrapolation)
 
-to the log-reluctivity the converged solution of the next step will have,
-learnt as a correction to log nu(|B_extrap|^2) (the B-H curve at the
-extrapolated field);
+as the 16 inputs (B1, Be, 10 dB, |B1|^2, |Be|^2, drive(t_n), 100 |dB|^2,
+|Bq|^2, region code; B in units of 2.2 T, |B|^2 of 4.84 T^2) to the
+log-reluctivity the converged solution of step n will have, learnt as a
+correction to log nu(|Be|^2) (the B-H curve at the extrapolated field);
 ``drive`` is an optional pair of global features of the prescribed source at
 the new time (for a sinusoidal winding: sin and cos of its phase).
 The warm start assembles K(nu_hat) + M_sigma/dt + reg M_reg (the Newton
@@ -64,17 +68,16 @@ class NuPredictor:
                         and v.Type == "Nonlinear")
         if not nonlin:
             raise ValueError("no nonlinear material: nothing to predict")
-        if len(nonlin) > N_IN - 13:
-            raise ValueError("too many nonlinear regions for the feature vector")
         self.tags = nonlin
         self.drive = drive
         tags = mesh.RegionTags
         sel = np.isin(tags, nonlin)
         self.idx = torch.from_numpy(np.nonzero(sel)[0].astype(np.int64)).to(self.dev)
-        onehot = np.zeros((int(sel.sum()), len(nonlin)), dtype=np.float32)
+        # region code of each selected element: (j + 1) / (number of regions)
+        code = np.zeros(int(sel.sum()), dtype=np.float32)
         for j, t in enumerate(nonlin):
-            onehot[tags[sel] == t, j] = 1.0
-        self.onehot = torch.from_numpy(onehot).to(self.dev)
+            code[tags[sel] == t] = (j + 1.0) / len(nonlin)
+        self.region = torch.from_numpy(code).to(self.dev)
         self.tags_sel = torch.from_numpy(tags[sel].astype(np.int32)).to(self.dev)
         self.mat, self._keep = assembler.material_tables(matLib)
         # reluctivity of the linear regions' elements (their constant nu)
@@ -91,27 +94,38 @@ class NuPredictor:
         self.wb = torch.empty(NBF16, dtype=torch.int16, device=self.dev)
         L.check(L.lib.mpk_mlp_sync_bf16(L.ptr(self.params), L.ptr(self.wb), L.stream_ptr()))
         self.X, self.T = [], []
+        # converged B of the selected elements, most recent first (zeros
+        # before the first steps: the transient starts from A = 0)
+        self.hist = [torch.zeros((self.idx.numel(), 3), dtype=torch.float64, device=self.dev)
+                     for _ in range(3)]
         self.t_mean, self.t_std = 0.0, 1.0
         self.adam_step = 0
         self.trained = False
         self.train_info = {}
 
     # ------------------------------------------------------------- features
-    def features(self, B_prev: torch.Tensor, B_prev2: torch.Tensor, t_next: float = 0.0):
-        bp = B_prev.index_select(0, self.idx)
-        bq = B_prev2.index_select(0, self.idx)
-        be = 2.0 * bp - bq
+    def observe(self, B: torch.Tensor) -> None:
+        """Records a converged step's B (all elements)."""
+        self.hist = [B.index_select(0, self.idx).to(torch.float64)] + self.hist[:2]
+
+    def features(self, t_next: float = 0.0) -> torch.Tensor:
+        b1, b2, b3 = self.hist
+        be = 2.0 * b1 - b2
+        db = b1 - (2.0 * b2 - b3)
+        bq = 3.0 * b1 - 3.0 * b2 + b3
         X = torch.zeros((self.idx.numel(), N_IN), dtype=torch.float32, device=self.dev)
-        X[:, 0:3] = (bp / B_SCALE).float()
+        X[:, 0:3] = (b1 / B_SCALE).float()
         X[:, 3:6] = (be / B_SCALE).float()
-        X[:, 6:9] = (bq / B_SCALE).float()
-        X[:, 9] = ((bp * bp).sum(dim=1) / S_SCALE).float()
+        X[:, 6:9] = (db * (10.0 / B_SCALE)).float()
+        X[:, 9] = ((b1 * b1).sum(dim=1) / S_SCALE).float()
         X[:, 10] = ((be * be).sum(dim=1) / S_SCALE).float()
         if self.drive is not None:
             d0, d1 = self.drive(t_next)
             X[:, 11] = float(d0)
             X[:, 12] = float(d1)
-        X[:, 13:13 + self.onehot.shape[1]] = self.onehot
+        X[:, 13] = ((db * db).sum(dim=1) * (100.0 / S_SCALE)).float()
+        X[:, 14] = ((bq * bq).sum(dim=1) / S_SCALE).float()
+        X[:, 15] = self.region
         return X
 
     def log_nu(self, B: torch.Tensor, selected: bool = False) -> torch.Tensor:
@@ -124,13 +138,16 @@ class NuPredictor:
                                   L.ptr(nu), None, L.stream_ptr()))
         return torch.log(nu)
 
-    def _base(self, B_prev, B_prev2) -> torch.Tensor:
-        bp = B_prev.index_select(0, self.idx)
-        return self.log_nu(2.0 * bp - B_prev2.index_select(0, self.idx), selected=True)
+    def _base(self) -> torch.Tensor:
+        """log nu at the extrapolated field 2 B1 - B2."""
+        return self.log_nu(2.0 * self.hist[0] - self.hist[1], selected=True)
 
-    def add_sample(self, B_prev, B_prev2, B_conv, t_next: float = 0.0):
-        self.X.append(self.features(B_prev, B_prev2, t_next))
-        self.T.append((self.log_nu(B_conv) - self._base(B_prev, B_prev2)).float())
+    def add_sample(self, B_conv: torch.Tensor, t_next: float = 0.0) -> None:
+        """Adds the (features, target) pair of a converged step to the training
+        set and records the step."""
+        self.X.append(self.features(t_next))
+        self.T.append((self.log_nu(B_conv) - self._base()).float())
+        self.observe(B_conv)
 
     # ------------------------------------------------------------- training
     def fit(self, iters: int = 20000, batch: int = 8192, lr: float = 5e-3, seed: int = 0,
@@ -183,39 +200,35 @@ class NuPredictor:
                                       X.shape[0], L.ptr(y), L.stream_ptr()))
         return y
 
-    def predict_nu(self, B_prev, B_prev2, t_next: float = 0.0) -> torch.Tensor:
-        """nu_hat for every element: the prediction in nonlinear regions, nu0 elsewhere."""
-        y = self._forward(self.features(B_prev, B_prev2, t_next))
+    def predict_nu(self, t_next: float = 0.0) -> torch.Tensor:
+        """nu_hat for every element at the next step: the prediction in
+        nonlinear regions, nu0 elsewhere."""
+        y = self._forward(self.features(t_next))
         nu = self.nu_lin.clone()
-        nu[self.idx] = torch.exp(self._base(B_prev, B_prev2) + y.double() * self.t_std +
-                                 self.t_mean)
+        nu[self.idx] = torch.exp(self._base() + y.double() * self.t_std + self.t_mean)
         return nu
 
 
 def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5,
                        eta_warm: float = 1e-2):
     """warm_start hook for TransientSolver.solve (None until the network is
-    trained).  With nu_hat predicted from B_prev and its extrapolation, it
-    solves the secant system A x = b, A = K(nu_hat) + M_sigma/dt + reg M_reg,
-    for the correction around the extrapolated state x_e = 2 x_prev - x_prev2:
+    trained).  With nu_hat predicted from the recorded flux history (the
+    predictor's on_step hook, training_collector, keeps it), it solves the
+    secant system A x = b, A = K(nu_hat) + M_sigma/dt + reg M_reg, for the
+    correction around the extrapolated state x_e = 2 x_prev - x_prev2:
     A d = b - A x_e, x0 = x_e + d, to relative accuracy eta_warm (never
     tighter than the step's Newton target, eta_floor * max(Tolerance,
     RelTolerance |R(x_prev)|) absolute): Newton polishes what the prediction
-    and the solve leave.  Tighter warm solves do not pay: even with nu_hat
-    within 0.05% the warm residual stays near 2e-3 of the step's first
-    residual, and Newton takes the same number of iterations.  Looser ones do
-    not either: a step's first solve needs ~400 PCG iterations for any
-    tolerance from 1e-2 to 0.3 on the E-core transient."""
-    asm = predictor.asm
+    and the solve leave.  Measured on a 240k-tet E-core transient, warm
+    tolerances of 3e-3 and 3e-2 take as many PCG iterations per step as 1e-2
+    or more."""
 
     def hook(ctx):
         if not predictor.trained:
             return None
         s = ctx["solver"]
         x_prev, x_prev2, b, dt = ctx["x_prev"], ctx["x_prev2"], ctx["b"], ctx["dt"]
-        bp = asm.computeElementB(x_prev, space)
-        bp2 = asm.computeElementB(x_prev2, space)
-        nu = predictor.predict_nu(bp, bp2, ctx["t"]).contiguous()
+        nu = predictor.predict_nu(ctx["t"]).contiguous()
         kv = torch.empty_like(s.jsys)
         L.check(L.lib.mpk_csc_curl_curl(s.plan.handle, None, L.ptr(nu), C.byref(s.qk),
                                         L.PREC_FP64, L.ptr(kv), s.sp))
@@ -238,25 +251,20 @@ def learned_warm_start(predictor: NuPredictor, space, eta_floor: float = 0.5,
 
 def training_collector(predictor: NuPredictor, space, train_steps: int, train_iters: int = 20000,
                        log=None):
-    """on_step hook for TransientSolver.solve: after each of the first
-    ``train_steps`` steps it adds the step's (features, converged log nu)
-    samples to ``predictor``, and after the last of them trains the network
-    (the warm-start hook stays inactive until then)."""
+    """on_step hook for TransientSolver.solve: records every converged step
+    in the predictor's history; over the first ``train_steps`` steps it adds
+    the (features, converged log nu) pairs to the training set, and after
+    the last of them trains the network (the warm-start hook stays inactive
+    until then)."""
     asm = predictor.asm
-    bcache = {}
 
     def on_step(ctx):
         k = ctx["step"]
-        if k > train_steps:
-            return
-        bp = bcache.get(k - 1)
-        if bp is None:
-            bp = asm.computeElementB(ctx["x_prev"], space)
-        bp2 = bcache.get(k - 2, bp)
         bx = asm.computeElementB(ctx["x"], space)
-        bcache[k] = bx
-        bcache.pop(k - 3, None)
-        predictor.add_sample(bp, bp2, bx, ctx["t"])
+        if k > train_steps:
+            predictor.observe(bx)
+            return
+        predictor.add_sample(bx, ctx["t"])
         if k == train_steps:
             info = predictor.fit(iters=train_iters)
             if log is not None:

@@ -175,6 +175,10 @@ class TransientSolver:
         self.LinearEtaMax = 1e-2
         self.LinearGamma = 0.9
         self.LinearEtaFloor = 0.5
+        # None: the forcing above (inexact Newton).  A number: every Newton
+        # step's linear system is solved to that relative residual, as the
+        # reference's direct solve (MUMPS) solves it exactly
+        self.NewtonLinearTolerance = None
         self.LinearMaxIter = 20000
         self.LinearCheckEvery = 10
         # solveAV: relative CG tolerance of the A-V Newton solves
@@ -663,10 +667,13 @@ class TransientSolver:
             target = max(self.Tolerance, self.RelTolerance * initial_res)
             # Eisenstat-Walker forcing, never tighter than what reaching the
             # target from this residual needs
-            eta = self.LinearEtaMax
-            if prev_norm is not None:
-                eta = min(eta, self.LinearGamma * (norm_res / prev_norm) ** 2)
-            eta = min(max(eta, self.LinearEtaFloor * target / norm_res), 0.5)
+            if self.NewtonLinearTolerance is not None:
+                eta = float(self.NewtonLinearTolerance)
+            else:
+                eta = self.LinearEtaMax
+                if prev_norm is not None:
+                    eta = min(eta, self.LinearGamma * (norm_res / prev_norm) ** 2)
+                eta = min(max(eta, self.LinearEtaFloor * target / norm_res), 0.5)
             dx, nit = self._linear_solve(-r, eta)
             info.pcg_iters += nit
             info.pcg_per_solve.append(nit)

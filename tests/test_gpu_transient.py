@@ -40,7 +40,7 @@ def test_transient_matches_cpu_restatement(gpu):
     assert np.array_equal(fixed, fixed_cpu)
     solver = TransientSolver(asm)
     # every Newton step's PCG to 1e-12, as the restatement's direct solves
-    solver.LinearEtaMax, solver.LinearGamma, solver.LinearEtaFloor = 1e-12, 1e30, 0.0
+    solver.NewtonLinearTolerance = 1e-12
     js = torch.from_numpy(ec.js_dir).cuda()
     sols, infos = solver.solve(space, lib, {2: 2e6}, lambda t: js * (amp * np.sin(2 * np.pi * 50 * t)),
                                steps, fixed)
