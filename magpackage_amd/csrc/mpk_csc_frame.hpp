@@ -64,6 +64,16 @@ __device__ __forceinline__ Vec3 ld_node(const double4* __restrict__ P4, int node
     return Vec3{xy.x, xy.y, z};
 }
 
+// Adds v to an LDS accumulator entry with one ds_add_f64 (no return value):
+// no read back and no wait, where `a += v` costs a read, a wait and a write.
+// Each entry belongs to one lane's column, so the additions to it happen in
+// that lane's program order, the order (and the rounding) of `a += v`.
+// Column tile, ring driver, 4M-tet Newton set: 0.718-0.725 -> 0.712-0.715 ms
+// FP32_LOCAL; K + j w M on the 128^3 box: 1.364 -> 1.330 ms.
+__device__ __forceinline__ void lds_add(double* a, double v) {
+    __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 struct FrameNodes {
     Vec3 x[4];
 };
@@ -165,7 +175,7 @@ struct FrameCurl {
         for (int rp = 0; rp < 6; ++rp) {
             const int p = fpos(pos, rp);
             const double d = curl_dot<kPrec>(g, rp);
-            if (p != 0xff) acc[p] += d * (scale * fsign(f, rp));
+            if (p != 0xff) lds_add(&acc[p], d * (scale * fsign(f, rp)));
         }
     }
 };
@@ -197,7 +207,7 @@ struct FrameMass {
 #pragma unroll
         for (int rp = 0; rp < 6; ++rp) {
             const int p = fpos(pos, rp);
-            if (p != 0xff) acc[p] += m[rp] * (scale * fsign(f, rp));
+            if (p != 0xff) lds_add(&acc[p], m[rp] * (scale * fsign(f, rp)));
         }
     }
 };
@@ -241,10 +251,8 @@ struct FrameKM {
             const double d = curl_dot<kPrec>(g, rp);
             const double s = fsign(f, rp);
             if (p != 0xff) {
-                Acc a = acc[p];
-                a.x += d * (ks * s);
-                if (has_m) a.y += m[rp] * (ms * s);
-                acc[p] = a;
+                lds_add(&acc[p].x, d * (ks * s));
+                if (has_m) lds_add(&acc[p].y, m[rp] * (ms * s));
             }
         }
     }
@@ -341,7 +349,7 @@ struct FrameJac {
                 const double BCr = two_inv * dot(Bv, g.u[rp]);
                 const double uu = curl_dot<kPrec>(g, rp);
                 const double val = nu * (cc * uu) + 2.0 * dnu * BCr * BC0;
-                if (p != 0xff) acc[p] += val * wd * (fsign(f, rp) * f0);
+                if (p != 0xff) lds_add(&acc[p], val * wd * (fsign(f, rp) * f0));
             }
         }
     }
@@ -415,7 +423,7 @@ struct FrameNewton {
 #pragma unroll
         for (int rp = 0; rp < 6; ++rp) {
             const int p = fpos(pos, rp);
-            if (p != 0xff) acc[p] += curl_dot<kPrec>(g, rp) * (ks * fsign(f, rp));
+            if (p != 0xff) lds_add(&acc[p], curl_dot<kPrec>(g, rp) * (ks * fsign(f, rp)));
         }
         if (has_m) {
             const double ms = sigma * inv * f0;
@@ -424,7 +432,7 @@ struct FrameNewton {
 #pragma unroll
             for (int rp = 0; rp < 6; ++rp) {
                 const int p = fpos(pos, rp);
-                if (p != 0xff) acc[stride + p] += m[rp] * (ms * fsign(f, rp));
+                if (p != 0xff) lds_add(&acc[stride + p], m[rp] * (ms * fsign(f, rp)));
             }
         }
         if (out_j && dnu != 0.0) {
@@ -433,7 +441,7 @@ struct FrameNewton {
             for (int rp = 0; rp < 6; ++rp) {
                 const int p = fpos(pos, rp);
                 if (p != 0xff)
-                    acc[kD * stride + p] += (two_inv * dot(Bv, g.u[rp])) * (js * fsign(f, rp));
+                    lds_add(&acc[kD * stride + p], (two_inv * dot(Bv, g.u[rp])) * (js * fsign(f, rp)));
             }
         }
     }
@@ -499,7 +507,7 @@ struct FrameNewtonLin {
 #pragma unroll
         for (int rp = 0; rp < 6; ++rp) {
             const int p = fpos(pos, rp);
-            if (p != 0xff) acc[p] += curl_dot<kPrec>(g, rp) * (ks * fsign(f, rp));
+            if (p != 0xff) lds_add(&acc[p], curl_dot<kPrec>(g, rp) * (ks * fsign(f, rp)));
         }
     }
 };
