@@ -49,6 +49,8 @@ __device__ __forceinline__ T sel6(int c, T v0, T v1, T v2, T v3, T v4, T v5) {
     return c == 0 ? v0 : c == 1 ? v1 : c == 2 ? v2 : c == 3 ? v3 : c == 4 ? v4 : v5;
 }
 
+// FMA contraction within one expression only (see mpk_csc_frame.hpp).
+#pragma clang fp contract(on)
 struct Vec3 {
     double x, y, z;
 };
@@ -191,15 +193,17 @@ __device__ __forceinline__ void tile_end(const Op& op, const TileRange& t, const
     __syncthreads();
     if constexpr (HasStoreTile<Op>::value) {
         op.store_tile(t.base, acc, t.size);
-        return;
-    }
-    for (int i = threadIdx.x; i < t.size; i += kW) {
-        if constexpr (Planes<Op>::value > 1)
-            op.store_planes(t.base + i, acc + i, t.size);
-        else
-            op.store(t.base + i, acc[i]);
+    } else {
+        for (int i = threadIdx.x; i < t.size; i += kW) {
+            if constexpr (Planes<Op>::value > 1)
+                op.store_planes(t.base + i, acc + i, t.size);
+            else
+                op.store(t.base + i, acc[i]);
+        }
     }
 }
+
+#pragma clang fp contract(fast)
 
 }  // namespace csc
 }  // namespace mpk

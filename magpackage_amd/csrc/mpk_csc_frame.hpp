@@ -28,6 +28,12 @@
 namespace mpk {
 namespace csc {
 
+// Closed-form arithmetic contracts a * b + c to an FMA only within one
+// expression (not across statements, as -ffp-contract=fast allows): the
+// rounding then follows the source alone, so two kernels that inline the
+// same functions (e.g. the K-only and the K + M passes) give the same bits.
+#pragma clang fp contract(on)
+
 constexpr int kRingBatch = 3;
 
 // In-frame row positions pos'_0..pos'_5 (bytes) of the 6 frame edges.
@@ -854,6 +860,8 @@ struct FrameKMF {
         km.compute(nodes, el, f, pos, acc, c);
     }
 };
+
+#pragma clang fp contract(fast)
 
 }  // namespace csc
 }  // namespace mpk
