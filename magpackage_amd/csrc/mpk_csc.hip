@@ -94,14 +94,22 @@ __device__ inline double2 put_state(float4* s0, float2* s1, int64_t e, double bx
 }
 
 // Zeroes n doubles with streaming (non-temporal) stores: 1024 per block,
-// coalesced 64-lane runs.
+// coalesced 64-lane runs of 16-byte stores.
 constexpr int kFillBlock = 256;
 __global__ void __launch_bounds__(kFillBlock) k_fill_zero(double* __restrict__ p, int64_t n) {
-    const int64_t b = (int64_t)blockIdx.x * (4 * kFillBlock) + threadIdx.x;
+    // pairs of doubles (16-byte stores) from the first 16-byte aligned entry
+    const int64_t a = (int64_t)((reinterpret_cast<uintptr_t>(p) >> 3) & 1u);
+    const int64_t npair = (n - a) >> 1;
+    double2* p2 = reinterpret_cast<double2*>(p + a);
+    const int64_t b = (int64_t)blockIdx.x * (2 * kFillBlock) + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int64_t i = b + (int64_t)k * kFillBlock;
-        if (i < n) st_out(p + i, 0.0);
+    for (int k = 0; k < 2; ++k) {
+        const int64_t q = b + (int64_t)k * kFillBlock;
+        if (q < npair) st_out(p2 + q, make_double2(0.0, 0.0));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (a && n > 0) st_out(p, 0.0);
+        if (n > a && ((n - a) & 1)) st_out(p + n - 1, 0.0);
     }
 }
 

@@ -179,6 +179,27 @@ __device__ __forceinline__ void st_out(double2* p, double2 v) {
 #endif
 }
 
+// Writes tile entries [0, size) of an 8-byte output array at out + base,
+// two entries per lane and store (16-byte stores) from the first 16-byte
+// aligned address on; val(i) is tile entry i.
+template <class F>
+__device__ __forceinline__ void store_run(double* __restrict__ out, int64_t base, int size,
+                                          F val) {
+    if (size <= 0) return;
+    double* o = out + base;
+    const int a = (int)((reinterpret_cast<uintptr_t>(o) >> 3) & 1u);  // entries before
+    const int npair = (size - a) >> 1;
+    double2* o2 = reinterpret_cast<double2*>(o + a);
+    for (int q = threadIdx.x; q < npair; q += kW) {
+        const int i = a + 2 * q;
+        st_out(o2 + q, make_double2(val(i), val(i + 1)));
+    }
+    if (threadIdx.x == 0) {
+        if (a) st_out(o, val(0));
+        if ((size - a) & 1) st_out(o + size - 1, val(size - 1));
+    }
+}
+
 // Ops may store a finished tile themselves (store_tile(base, acc, size)),
 // e.g. to order the stores of several output arrays.
 template <class Op, class = void>

@@ -171,6 +171,9 @@ struct FrameCurl {
     __device__ Col col_begin() const { return Col{}; }
     __device__ void col_end(int64_t, Col&) const {}
     __device__ void store(int64_t p, Acc a) const { st_out(out + p, a); }
+    __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
+        store_run(out, base, size, [&](int i) { return acc[i]; });
+    }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return nu[e]; }
     __device__ void compute(const FrameNodes& nodes, const Elem& nuv, uint32_t f, const FPos& pos,
@@ -201,6 +204,9 @@ struct FrameMass {
     __device__ Col col_begin() const { return Col{}; }
     __device__ void col_end(int64_t, Col&) const {}
     __device__ void store(int64_t p, Acc a) const { st_out(out + p, a); }
+    __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
+        store_run(out, base, size, [&](int i) { return acc[i]; });
+    }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return coeffs ? coeffs[e] : 1.0; }
     __device__ void compute(const FrameNodes& nodes, const Elem& sigma, uint32_t f,
@@ -333,6 +339,9 @@ struct FrameJac {
     __device__ void store(int64_t p, Acc a) const {
         if (kCalcJ) st_out(out + p, a);
     }
+    __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
+        if (kCalcJ) store_run(out, base, size, [&](int i) { return acc[i]; });
+    }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
     __device__ void compute(const FrameNodes& nodes, const Elem& raw, uint32_t f,
@@ -401,13 +410,10 @@ struct FrameNewton {
     }
     // array-outer: each output's tile range is written as one contiguous run
     __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
-        if (out_k)
-            for (int i = threadIdx.x; i < size; i += kW) st_out(out_k + base + i, acc[i]);
-        if (kWithM)
-            for (int i = threadIdx.x; i < size; i += kW) st_out(out_m + base + i, acc[size + i]);
+        if (out_k) store_run(out_k, base, size, [&](int i) { return acc[i]; });
+        if (kWithM) store_run(out_m, base, size, [&](int i) { return acc[size + i]; });
         if (out_j)
-            for (int i = threadIdx.x; i < size; i += kW)
-                st_out(out_j + base + i, acc[i] + acc[kD * size + i]);
+            store_run(out_j, base, size, [&](int i) { return acc[i] + acc[kD * size + i]; });
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load(e); }
@@ -488,12 +494,9 @@ struct FrameNewtonLin {
     }
     // array-outer: each output's tile range is written as one contiguous run
     __device__ void store_tile(int64_t base, const Acc* acc, int size) const {
-        if (out_k)
-            for (int i = threadIdx.x; i < size; i += kW) st_out(out_k + base + i, acc[i]);
-        if (out_m)
-            for (int i = threadIdx.x; i < size; i += kW) st_out(out_m + base + i, 0.0);
-        if (out_j)
-            for (int i = threadIdx.x; i < size; i += kW) st_out(out_j + base + i, acc[i] + 0.0);
+        if (out_k) store_run(out_k, base, size, [&](int i) { return acc[i]; });
+        if (out_m) store_run(out_m, base, size, [&](int) { return 0.0; });
+        if (out_j) store_run(out_j, base, size, [&](int i) { return acc[i] + 0.0; });
     }
     __device__ Ids ids(const TileCtx&, int, const FPos&) const { return Ids{}; }
     __device__ Elem elem(uint32_t e, const Ids&) const { return st.load_bnu(e); }

@@ -157,7 +157,7 @@ RUNS = {"dt1e-3": ([1e-3] * 100, 8.0, 1e-4, 24, None, 1500.0, 3000, 2),
         "r02_long_tol1e-2": ([1e-3] * 100, 0.2, 1e-2, 23, None, 1500.0, 3000, 2),
         # the first 300 of 20000 recorded steps
         "r02_long_dt1e-4": ([1e-4] * 300, 0.2, 1e-3, 23, None, 1500.0, 3000, 2)}
-RUN_BOUNDS = {"r10": 1e-5, "r100": 1e-3}
+RUN_BOUNDS = {"r10": 1e-4, "r100": 1e-3}
 
 
 @pytest.mark.parametrize("run", list(RUNS))
@@ -214,10 +214,13 @@ def test_ecore_inrush_against_recorded_run(gpu, run):
           f"I {cur[:3]} vs {i_ref[:3]}; Newton total {sum(solver.NewtonIterations)}, "
           f"PCG total {sum(solver.PcgIterations)}", flush=True)
     # bounds over the measured deviations: ~1e-9 where the recording's Newton
-    # tolerance is 1e-4 or the steps converge well below it; 2e-6 (r10) and
-    # 4e-4 (r100) where two Newton iterations per step stop at 1e-3, so a
-    # residual near the threshold stops one iteration apart.  In the
-    # 100-ohm runs ~97% of the source voltage is induced, so flux errors
+    # tolerance is 1e-4 or the steps converge well below it; 2e-6 - 2.5e-5
+    # (r10) and 4e-4 (r100) where two Newton iterations per step stop at 1e-3,
+    # so a residual near the threshold stops one iteration apart (which step
+    # does so follows the last bits of the assembled matrices: 2e-6 and
+    # 2.5e-5 were measured with two FMA-contraction choices of the same
+    # kernels), still far inside the recording's 1e-3 Newton tolerance.  In
+    # the 100-ohm runs ~97% of the source voltage is induced, so flux errors
     # reach the current amplified ~30 times.
     bound = RUN_BOUNDS.get(run, 1e-7)
     assert np.abs(cur[:2] - i_ref[:2]).max() <= 1e-7 * np.abs(i_ref[:2]).max()
