@@ -94,9 +94,18 @@ __device__ inline double2 put_state(float4* s0, float2* s1, int64_t e, double bx
 }
 
 // Zeroes n doubles with streaming (non-temporal) stores: 1024 per block,
-// coalesced 64-lane runs of 16-byte stores.
+// coalesced 64-lane runs of 16-byte stores.  With flag given, thread i of the
+// grid also clears flag[i] (i < ntiles) and thread 0 the count (the work of
+// k_clear_tiles, so the Newton set needs no launch of its own for it).
 constexpr int kFillBlock = 256;
-__global__ void __launch_bounds__(kFillBlock) k_fill_zero(double* __restrict__ p, int64_t n) {
+__global__ void __launch_bounds__(kFillBlock)
+k_fill_zero(double* __restrict__ p, int64_t n, int32_t* __restrict__ flag = nullptr,
+            int64_t ntiles = 0, int32_t* __restrict__ count = nullptr) {
+    if (flag) {
+        const int64_t t = (int64_t)blockIdx.x * kFillBlock + threadIdx.x;
+        if (t < ntiles) flag[t] = 0;
+        if (t == 0) *count = 0;
+    }
     // pairs of doubles (16-byte stores) from the first 16-byte aligned entry
     const int64_t a = (int64_t)((reinterpret_cast<uintptr_t>(p) >> 3) & 1u);
     const int64_t npair = (n - a) >> 1;
@@ -553,32 +562,38 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
             flags = plan->buf_tile_flag.as<int32_t>();
         }
     }
-    // M is zero on the tiles no M term reaches.  A streaming fill on the
-    // plan's side stream writes zeros over the whole M array while the
-    // element state and the class-0 pass run; the class-0 pass then stores K
-    // and J only, and the listed pass, ordered after the fill, writes its
-    // tiles' M (4M-tet Newton workload: 0.738-0.747 -> 0.720 ms FP32_LOCAL,
-    // 0.775-0.780 -> 0.764-0.767 ms FP64).  MPK_NEWTON_MFILL=0 stores the
-    // zeros from the class-0 pass instead; the allocation-free entry point
-    // always does (it creates no stream).
+    // M is zero on the tiles no M term reaches.  A streaming fill writes
+    // zeros over the whole M array before the element state; the class-0
+    // pass then stores K and J only, and the listed pass writes its tiles' M
+    // (4M-tet Newton workload: 0.738-0.747 -> 0.720 ms FP32_LOCAL; run on
+    // the caller's stream rather than beside the element state on a side
+    // stream, which needed fork and join events: 0.691 -> 0.680 ms).
+    // MPK_NEWTON_MFILL=0 stores the zeros from the class-0 pass instead; the
+    // allocation-free entry point always does.
     bool mfill = split && with_m && !noalloc && plan->nnz > 0;
     if (const char* e = std::getenv("MPK_NEWTON_MFILL")) mfill = mfill && std::atoi(e) != 0;
-    if (mfill) {
-        MPK_HIP_TRY(plan->side.ensure());
-        MPK_HIP_TRY(hipEventRecord(plan->side.fork, st));
-        MPK_HIP_TRY(hipStreamWaitEvent(plan->side.s, plan->side.fork, 0));
-        hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(plan->nnz, 4 * kFillBlock)),
-                           dim3(kFillBlock), 0, plan->side.s, m_vals, plan->nnz);
-        MPK_HIP_TRY(hipGetLastError());
-    }
     // The classes depend on sigma and dnu, not on which outputs are requested
     // (a call for a subset of the outputs runs each tile with the same op as
     // the full call).  Without sigma the state holds sigma = 1: M then reaches
     // every tile.
     SV sv{};
-    if (int rc = elem_state(plan, P, tags, sol_a, sigma, mat, st, &sv, flags,
-                            with_m || sigma != nullptr, true, scratch_state))
+    ElemStateRun<SV> esr;
+    if (int rc = elem_state_prepare(plan, P, tags, sol_a, sigma, mat, st, &sv, flags,
+                                    with_m || sigma != nullptr, true, scratch_state, &esr))
         return rc;
+    if (mfill) {
+        // the fill also clears the tile classes (flags is set: mfill needs split)
+        const unsigned g1 = grid_for(plan->nnz, 4 * kFillBlock);
+        const unsigned g2 = grid_for(ntiles, kFillBlock);
+        hipLaunchKernelGGL(k_fill_zero, dim3(g1 > g2 ? g1 : g2), dim3(kFillBlock), 0, st, m_vals,
+                           plan->nnz, flags, ntiles, flags + 2 * ntiles);
+        MPK_HIP_TRY(hipGetLastError());
+    }
+    if (esr.plan) {
+        if (!mfill) MPK_HIP_TRY(esr.clear(st));
+        MPK_HIP_TRY(esr.run(0, plan->ne, st));
+        MPK_HIP_TRY(esr.list(st));
+    }
     if (split) {
         const int32_t* list = flags + ntiles;
         const int32_t* count = flags + 2 * ntiles;
@@ -619,10 +634,6 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
         };
         if (!overlap) {
             if (int rc = class0()) return rc;
-            if (mfill) {
-                MPK_HIP_TRY(hipEventRecord(plan->side.join, plan->side.s));
-                MPK_HIP_TRY(hipStreamWaitEvent(st, plan->side.join, 0));
-            }
             return listed(st);
         }
         // the listed tiles on the side stream, the class-0 pass on the
