@@ -1,0 +1,241 @@
+/*
+ * san_check.c -- drives the CPU oracle (mpk_oracle.c) on a small jittered
+ * Kuhn box under AddressSanitizer and UndefinedBehaviorSanitizer
+ * (`make -C oracle sanitize`).  TEST INFRASTRUCTURE ONLY, like the oracle.
+ *
+ * Every oracle function the parity tests use runs once on in-bounds inputs:
+ * edges, DOFs, the curl-curl, mass, Jacobian (linear and nonlinear tags),
+ * source and element-B kernels, sparse() and the boundary edges.  Beside the
+ * sanitizers' own checks it asserts what the mathematics fixes:
+ *   - the Kuhn box's edge count 3n(n+1)^2 + 3n^2(n+1) + n^3;
+ *   - sparse(K) is symmetric;
+ *   - for a linear material the Jacobian equals K and R = K a.
+ * Exit status 0 when every check holds.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+int64_t orc_generate_edges(const int32_t* T, int64_t ne, int32_t* edges_out, int32_t* t2e_out,
+                           int8_t* sign_out);
+int64_t orc_distribute_dofs(const int32_t* t2e, int64_t ne, int64_t nedge,
+                            const uint8_t* active_elem, int64_t offset, int32_t* entity_map,
+                            int32_t* cell_dofs);
+void orc_curl_curl_kernel(const double* P, const double* T, const double* Dofs,
+                          const double* Signs, const double* Nu, const double* Qw, int64_t nq,
+                          const double* RefCurl, int64_t ne, double* I, double* Jo, double* V);
+void orc_mass_kernel(const double* P, const double* T, const double* Dofs, const double* Signs,
+                     const double* Coeffs, int64_t ncoeffs, const double* Qw, int64_t nq,
+                     const double* RefVal, int64_t ne, double* I, double* Jo, double* V);
+void orc_jacobian_kernel(const double* P, const double* T, const double* Dofs,
+                         const double* Signs, const double* Tags, const double* SolA,
+                         const double* Qw, int64_t nq, const double* RefCurl, const double* lin,
+                         const double* isNon, const double* maxB, const double* brk,
+                         const double* coe, const double* start, const double* cnt,
+                         const double* coeStart, int calcJ, int64_t ne, double* I, double* Jo,
+                         double* V, double* Ridx, double* Rval);
+void orc_source_kernel(const double* P, const double* T, const double* Dofs, const double* Signs,
+                       const double* Js, const double* Qw, int64_t nq, const double* RefVal,
+                       int64_t ne, double* I, double* V);
+void orc_element_b(const double* P, const double* T, const double* Dofs, const double* Signs,
+                   const double* sol, const double* CurlRef, int64_t ne, double* B);
+int64_t orc_sparse(const double* I, const double* J, const double* V, int64_t nt, int64_t m,
+                   int64_t n, int drop_zeros, int64_t* colptr, int64_t* rowind, double* vals);
+void orc_boundary_edges(const int32_t* T, int64_t ne, const int32_t* edges, int64_t nedge,
+                        uint8_t* is_bnd_edge);
+
+static int g_fail = 0;
+#define CHECK(c, ...)                        \
+    do {                                     \
+        if (!(c)) {                          \
+            fprintf(stderr, "FAIL: " __VA_ARGS__); \
+            fprintf(stderr, "\n");           \
+            g_fail = 1;                      \
+        }                                    \
+    } while (0)
+
+static uint64_t g_rng = 0x9e3779b97f4a7c15ull;
+static double urand(void) { /* xorshift64*, uniform in [0, 1) */
+    g_rng ^= g_rng >> 12;
+    g_rng ^= g_rng << 25;
+    g_rng ^= g_rng >> 27;
+    return (double)((g_rng * 0x2545f4914f6cdd1dull) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static void* xcalloc(size_t n, size_t sz) {
+    void* p = calloc(n ? n : 1, sz);
+    if (!p) {
+        fprintf(stderr, "out of memory\n");
+        exit(2);
+    }
+    return p;
+}
+
+/* CSC value at (r, c), 0 when not stored. */
+static double csc_at(const int64_t* cp, const int64_t* ri, const double* v, int64_t r, int64_t c) {
+    for (int64_t k = cp[c]; k < cp[c + 1]; k++)
+        if (ri[k] == r) return v[k];
+    return 0.0;
+}
+
+int main(void) {
+    const int n = 3, nn1 = n + 1;
+    const int64_t nnode = (int64_t)nn1 * nn1 * nn1, ne = 6LL * n * n * n;
+    double* P = xcalloc((size_t)(3 * nnode), sizeof(double));
+    for (int k = 0; k < nn1; k++)
+        for (int j = 0; j < nn1; j++)
+            for (int i = 0; i < nn1; i++) {
+                const int64_t id = i + (int64_t)nn1 * (j + (int64_t)nn1 * k);
+                const int idx[3] = {i, j, k};
+                for (int a = 0; a < 3; a++) {
+                    const int interior = idx[a] > 0 && idx[a] < n;
+                    const double jit = interior ? 0.2 * (urand() - 0.5) : 0.0;
+                    P[3 * id + a] = (idx[a] + jit) / n;
+                }
+            }
+    /* Kuhn split: the 6 paths from corner 000 to 111 along the axes */
+    static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    int32_t* Ti = xcalloc((size_t)(4 * ne), sizeof(int32_t));
+    int64_t e = 0;
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++)
+                for (int p = 0; p < 6; p++, e++) {
+                    int c[3] = {i, j, k};
+                    for (int v = 0; v < 4; v++) {
+                        if (v > 0) c[perm[p][v - 1]]++;
+                        Ti[4 * e + v] = 1 + c[0] + nn1 * (c[1] + nn1 * c[2]);
+                    }
+                }
+    double* T = xcalloc((size_t)(4 * ne), sizeof(double));
+    for (int64_t q = 0; q < 4 * ne; q++) T[q] = Ti[q];
+
+    int32_t* edges = xcalloc((size_t)(12 * ne), sizeof(int32_t));
+    int32_t* t2e = xcalloc((size_t)(6 * ne), sizeof(int32_t));
+    int8_t* sg8 = xcalloc((size_t)(6 * ne), sizeof(int8_t));
+    const int64_t nedge = orc_generate_edges(Ti, ne, edges, t2e, sg8);
+    const int64_t expect = 3LL * n * nn1 * nn1 + 3LL * n * n * nn1 + (int64_t)n * n * n;
+    CHECK(nedge == expect, "edges %lld, expected %lld", (long long)nedge, (long long)expect);
+
+    int32_t* emap = xcalloc((size_t)nedge, sizeof(int32_t));
+    int32_t* cell = xcalloc((size_t)(6 * ne), sizeof(int32_t));
+    const int64_t ndof = orc_distribute_dofs(t2e, ne, nedge, NULL, 0, emap, cell);
+    CHECK(ndof == nedge, "dofs %lld, edges %lld", (long long)ndof, (long long)nedge);
+    double* Dofs = xcalloc((size_t)(6 * ne), sizeof(double));
+    double* Signs = xcalloc((size_t)(6 * ne), sizeof(double));
+    for (int64_t q = 0; q < 6 * ne; q++) {
+        Dofs[q] = cell[q];
+        Signs[q] = sg8[q];
+    }
+
+    /* one-point rule at the centroid; Nedelec reference curls and values
+       (RefX[i + 6 a + 18 q]) */
+    const double Qw[1] = {1.0 / 6.0};
+    static const double curl[6][3] = {{0, -2, 2}, {2, 0, -2}, {-2, 2, 0},
+                                      {0, 0, 2},  {0, -2, 0}, {2, 0, 0}};
+    static const double gl[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    static const int ea[6] = {0, 0, 0, 1, 1, 2}, eb[6] = {1, 2, 3, 2, 3, 3};
+    double RefCurl[18], RefVal[18];
+    for (int i = 0; i < 6; i++)
+        for (int a = 0; a < 3; a++) {
+            RefCurl[i + 6 * a] = curl[i][a];
+            RefVal[i + 6 * a] = 0.25 * (gl[eb[i]][a] - gl[ea[i]][a]);
+        }
+
+    double* nu = xcalloc((size_t)ne, sizeof(double));
+    double* sigma = xcalloc((size_t)ne, sizeof(double));
+    double* tags = xcalloc((size_t)ne, sizeof(double));
+    for (int64_t q = 0; q < ne; q++) {
+        nu[q] = 1.0 + urand();
+        sigma[q] = (q % 3 == 0) ? 2.0 : 0.0;
+        tags[q] = 1.0 + (double)(q % 2);
+    }
+    double* I = xcalloc((size_t)(36 * ne), sizeof(double));
+    double* J = xcalloc((size_t)(36 * ne), sizeof(double));
+    double* V = xcalloc((size_t)(36 * ne), sizeof(double));
+    orc_curl_curl_kernel(P, T, Dofs, Signs, nu, Qw, 1, RefCurl, ne, I, J, V);
+    int64_t* cp = xcalloc((size_t)(ndof + 1), sizeof(int64_t));
+    int64_t* ri = xcalloc((size_t)(36 * ne), sizeof(int64_t));
+    double* kv = xcalloc((size_t)(36 * ne), sizeof(double));
+    const int64_t nnz = orc_sparse(I, J, V, 36 * ne, ndof, ndof, 0, cp, ri, kv);
+    CHECK(nnz > 0, "sparse returned %lld", (long long)nnz);
+    double kmax = 0.0, asym = 0.0;
+    for (int64_t c = 0; c < ndof && nnz > 0; c++)
+        for (int64_t k = cp[c]; k < cp[c + 1]; k++) {
+            kmax = fmax(kmax, fabs(kv[k]));
+            asym = fmax(asym, fabs(kv[k] - csc_at(cp, ri, kv, c, ri[k])));
+        }
+    CHECK(asym <= 1e-12 * kmax, "K asymmetry %.3e of %.3e", asym, kmax);
+
+    double* Im = xcalloc((size_t)(36 * ne), sizeof(double));
+    double* Jm = xcalloc((size_t)(36 * ne), sizeof(double));
+    double* Vm = xcalloc((size_t)(36 * ne), sizeof(double));
+    orc_mass_kernel(P, T, Dofs, Signs, sigma, ne, Qw, 1, RefVal, ne, Im, Jm, Vm);
+
+    /* material tables: tag 1 linear (nu0 = 1), tag 2 a 3-break cubic spline */
+    const double lin[2] = {1.0, 2.0}, isNon[2] = {0.0, 1.0}, maxB[2] = {0.0, 4.0};
+    const double brk[3] = {0.0, 1.0, 4.0};
+    const double coe[12] = {0.0, 0.01, 0.05, 0.5, 0.0, 0.0, 0.08, 0.56, 0.0, 0.0, 0.08, 0.80};
+    const double start[2] = {0.0, 0.0}, cnt[2] = {0.0, 3.0}, coeStart[2] = {0.0, 0.0};
+    double* a = xcalloc((size_t)ndof, sizeof(double));
+    for (int64_t q = 0; q < ndof; q++) a[q] = urand() - 0.5;
+    double* Ij = xcalloc((size_t)(36 * ne), sizeof(double));
+    double* Jj = xcalloc((size_t)(36 * ne), sizeof(double));
+    double* Vj = xcalloc((size_t)(36 * ne), sizeof(double));
+    double* Ridx = xcalloc((size_t)(6 * ne), sizeof(double));
+    double* Rval = xcalloc((size_t)(6 * ne), sizeof(double));
+    orc_jacobian_kernel(P, T, Dofs, Signs, tags, a, Qw, 1, RefCurl, lin, isNon, maxB, brk, coe,
+                        start, cnt, coeStart, 1, ne, Ij, Jj, Vj, Ridx, Rval);
+    /* all-linear tags: J = K(nu = 1) and R = K a */
+    double* ones = xcalloc((size_t)ne, sizeof(double));
+    double* tag1 = xcalloc((size_t)ne, sizeof(double));
+    for (int64_t q = 0; q < ne; q++) ones[q] = tag1[q] = 1.0;
+    orc_jacobian_kernel(P, T, Dofs, Signs, tag1, a, Qw, 1, RefCurl, lin, isNon, maxB, brk, coe,
+                        start, cnt, coeStart, 1, ne, Ij, Jj, Vj, Ridx, Rval);
+    orc_curl_curl_kernel(P, T, Dofs, Signs, ones, Qw, 1, RefCurl, ne, I, J, V);
+    double jdiff = 0.0, vmax = 0.0;
+    for (int64_t q = 0; q < 36 * ne; q++) {
+        jdiff = fmax(jdiff, fabs(Vj[q] - V[q]));
+        vmax = fmax(vmax, fabs(V[q]));
+    }
+    CHECK(jdiff <= 1e-12 * vmax, "linear Jacobian vs K: %.3e of %.3e", jdiff, vmax);
+    double* Ra = xcalloc((size_t)ndof, sizeof(double));
+    double* Ka = xcalloc((size_t)ndof, sizeof(double));
+    for (int64_t q = 0; q < 6 * ne; q++)
+        if (Ridx[q] > 0) Ra[(int64_t)Ridx[q] - 1] += Rval[q];
+    for (int64_t q = 0; q < 36 * ne; q++)
+        if (I[q] > 0 && J[q] > 0) Ka[(int64_t)I[q] - 1] += V[q] * a[(int64_t)J[q] - 1];
+    double rdiff = 0.0, rmax = 0.0;
+    for (int64_t q = 0; q < ndof; q++) {
+        rdiff = fmax(rdiff, fabs(Ra[q] - Ka[q]));
+        rmax = fmax(rmax, fabs(Ka[q]));
+    }
+    CHECK(rdiff <= 1e-12 * rmax, "R vs K a: %.3e of %.3e", rdiff, rmax);
+
+    double* Js = xcalloc((size_t)(3 * ne), sizeof(double));
+    for (int64_t q = 0; q < 3 * ne; q++) Js[q] = urand();
+    double* Is = xcalloc((size_t)(6 * ne), sizeof(double));
+    double* Vs = xcalloc((size_t)(6 * ne), sizeof(double));
+    orc_source_kernel(P, T, Dofs, Signs, Js, Qw, 1, RefVal, ne, Is, Vs);
+    double* B = xcalloc((size_t)(3 * ne), sizeof(double));
+    orc_element_b(P, T, Dofs, Signs, a, RefCurl, ne, B);
+
+    uint8_t* bnd = xcalloc((size_t)nedge, 1);
+    orc_boundary_edges(Ti, ne, edges, nedge, bnd);
+    int64_t nb = 0;
+    for (int64_t q = 0; q < nedge; q++) nb += bnd[q];
+    /* the box surface: 6 faces of n x n squares, each with 2 triangles */
+    const int64_t expect_b = 6LL * (2LL * n * nn1 + (int64_t)n * n) - 12LL * n;
+    CHECK(nb == expect_b, "boundary edges %lld, expected %lld", (long long)nb, (long long)expect_b);
+
+    printf("oracle sanitizer check: %lld tets, %lld edges, nnz %lld, %s\n", (long long)ne,
+           (long long)nedge, (long long)nnz, g_fail ? "FAILED" : "all checks passed");
+    free(P); free(Ti); free(T); free(edges); free(t2e); free(sg8); free(emap); free(cell);
+    free(Dofs); free(Signs); free(nu); free(sigma); free(tags); free(I); free(J); free(V);
+    free(cp); free(ri); free(kv); free(Im); free(Jm); free(Vm); free(a); free(Ij); free(Jj);
+    free(Vj); free(Ridx); free(Rval); free(ones); free(tag1); free(Ra); free(Ka); free(Js);
+    free(Is); free(Vs); free(B); free(bnd);
+    return g_fail;
+}
