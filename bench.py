@@ -18,8 +18,12 @@ line carries, under extra keys:
   nonlinear_step_time the E-core transient (983,040 tets), 200 backward-Euler
                       steps of 1 ms: plain Newton, Newton from the extrapolated
                       start and the learned warm start trained on steps 1-40,
-                      over steps 41-200 (inexact Newton; --workload ecore
-                      --newton-linear-tol 1e-10 gives the exact-solve regime).
+                      over steps 41-200 (inexact Newton);
+  nonlinear_step_time_exact_solve
+                      the same transient with every Newton linear system
+                      solved to 1e-10 (the reference's direct-solve regime),
+                      80 steps: plain Newton against the learned warm start
+                      over steps 41-80.
 
 Prints ONE JSON line (rank 0).  The timed region is exactly --steps launches
 between two device synchronisations; the kernel duration is measured with
@@ -574,6 +578,8 @@ def main():
                          "of the Eisenstat-Walker forcing")
     ap.add_argument("--no-nonlinear", action="store_true",
                     help="freq128: skip the 200-step learned-vs-plain nonlinear transient")
+    ap.add_argument("--no-exact", action="store_true",
+                    help="freq128: skip the exact-solve-regime learned-vs-plain transient")
     ap.add_argument("--no-newton", action="store_true",
                     help="freq128: skip the fused Newton set on the 100x100x67 box")
     ap.add_argument("--raw-source", action="store_true",
@@ -738,6 +744,24 @@ def main():
             nonlinear[kind] = {k: r[kind][k] for k in ("ms_per_step", "newton", "pcg", "converged")}
         nonlinear["learned"]["train"] = r["learned"]["train"]
         nonlinear["learned"]["warm_used"] = r["learned"]["warm_used"]
+        torch.cuda.empty_cache()
+
+    exact = None
+    if not args.no_nonlinear and not args.no_exact:
+        # the same transient in the reference's exact-solve regime (every
+        # Newton step's linear system solved to 1e-10, as its direct solve
+        # does): 80 steps, trained on steps 1-40, plain Newton and the learned
+        # warm start over steps 41-80 (the 200-step run takes ~8 minutes:
+        # python bench.py --workload ecore --dt 1e-3 --newton-linear-tol 1e-10)
+        ea = argparse.Namespace(**vars(args))
+        ea.steps, ea.train_steps, ea.dt = 80, 40, 1e-3
+        ea.only, ea.no_extrap_run, ea.verbose_newton = None, True, False
+        ea.newton_linear_tol = 1e-10
+        r = run_ecore(ea, emit=False)
+        exact = {k: r[k] for k in ("eval_steps", "speedup_learned_vs_plain", "config")}
+        for kind in ("plain", "learned"):
+            exact[kind] = {k: r[kind][k] for k in ("ms_per_step", "newton", "pcg", "converged")}
+        exact["learned"]["warm_used"] = r["learned"]["warm_used"]
 
     workload = f"kuhn{args.n}_freq_k_jwsm_c128"
     traffic = read_traffic(workload) if args.n == 128 else None
@@ -776,6 +800,7 @@ def main():
         "fp32_local": fp32,
         "newton_set": newton,
         "nonlinear_step_time": nonlinear,
+        "nonlinear_step_time_exact_solve": exact,
     }
     print(json.dumps(out), flush=True)
 
