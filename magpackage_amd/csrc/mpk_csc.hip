@@ -93,11 +93,12 @@ __device__ inline double2 put_state(float4* s0, float2* s1, int64_t e, double bx
     return make_double2(v.x, v.y);
 }
 
-// Zeroes n doubles with streaming (non-temporal) stores: 1024 per block,
-// coalesced 64-lane runs of 16-byte stores.  With flag given, thread i of the
-// grid also clears flag[i] (i < ntiles) and thread 0 the count (the work of
-// k_clear_tiles, so the Newton set needs no launch of its own for it).
+// Zeroes n doubles with streaming (non-temporal) stores: kFillPairs per
+// thread, coalesced 64-lane runs of 16-byte stores.  With flag given, thread i
+// of the grid also clears flag[i] (i < ntiles) and thread 0 the count (the
+// work of k_clear_tiles, so the Newton set needs no launch of its own for it).
 constexpr int kFillBlock = 256;
+constexpr int kFillPairs = 2;  // 16-byte stores per thread
 __global__ void __launch_bounds__(kFillBlock)
 k_fill_zero(double* __restrict__ p, int64_t n, int32_t* __restrict__ flag = nullptr,
             int64_t ntiles = 0, int32_t* __restrict__ count = nullptr) {
@@ -110,9 +111,9 @@ k_fill_zero(double* __restrict__ p, int64_t n, int32_t* __restrict__ flag = null
     const int64_t a = (int64_t)((reinterpret_cast<uintptr_t>(p) >> 3) & 1u);
     const int64_t npair = (n - a) >> 1;
     double2* p2 = reinterpret_cast<double2*>(p + a);
-    const int64_t b = (int64_t)blockIdx.x * (2 * kFillBlock) + threadIdx.x;
+    const int64_t b = (int64_t)blockIdx.x * (kFillPairs * kFillBlock) + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kFillPairs; ++k) {
         const int64_t q = b + (int64_t)k * kFillBlock;
         if (q < npair) st_out(p2 + q, make_double2(0.0, 0.0));
     }
@@ -581,9 +582,26 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     if (int rc = elem_state_prepare(plan, P, tags, sol_a, sigma, mat, st, &sv, flags,
                                     with_m || sigma != nullptr, true, scratch_state, &esr))
         return rc;
+    // Schedule, by measurement on the 4M-tet Newton workload: with the FP64
+    // state the listed pass runs best beside the class-0 pass, on the plan's
+    // side stream with a few persistent blocks per CU (0.90 -> 0.84 ms); with
+    // the FP32 state, after it with the full persistent grid (0.81 ms; side by
+    // side 0.84-1.07).  MPK_NEWTON_OVERLAP=0/1 and MPK_LIST_BLOCKS_PER_CU
+    // (1..16) override.
+    bool overlap = kPrec == MPK_PREC_FP64;
+    if (const char* e = std::getenv("MPK_NEWTON_OVERLAP")) overlap = std::atoi(e) != 0;
+    size_t per_cu = overlap ? kSidePerCu : 16;
+    if (const char* e = std::getenv("MPK_LIST_BLOCKS_PER_CU")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= 16) per_cu = (size_t)v;
+    }
+    if (noalloc) {
+        overlap = false;
+        per_cu = 16;
+    }
     if (mfill) {
         // the fill also clears the tile classes (flags is set: mfill needs split)
-        const unsigned g1 = grid_for(plan->nnz, 4 * kFillBlock);
+        const unsigned g1 = grid_for(plan->nnz, 2 * kFillPairs * kFillBlock);
         const unsigned g2 = grid_for(ntiles, kFillBlock);
         hipLaunchKernelGGL(k_fill_zero, dim3(g1 > g2 ? g1 : g2), dim3(kFillBlock), 0, st, m_vals,
                            plan->nnz, flags, ntiles, flags + 2 * ntiles);
@@ -592,28 +610,14 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     if (esr.plan) {
         if (!mfill) MPK_HIP_TRY(esr.clear(st));
         MPK_HIP_TRY(esr.run(0, plan->ne, st));
-        MPK_HIP_TRY(esr.list(st));
+        // The listed pass needs its list before it starts: beside the class-0
+        // pass, from a kernel of its own; after it, the class-0 pass's first
+        // blocks build it (one launch fewer: 0.678-0.681 -> 0.675-0.678 ms).
+        if (overlap) MPK_HIP_TRY(esr.list(st));
     }
     if (split) {
         const int32_t* list = flags + ntiles;
         const int32_t* count = flags + 2 * ntiles;
-        // Schedule, by measurement on the 4M-tet Newton workload: with the
-        // FP64 state the listed pass runs best beside the class-0 pass, on the
-        // plan's side stream with a few persistent blocks per CU (0.90 -> 0.84
-        // ms); with the FP32 state, after it with the full persistent grid
-        // (0.81 ms; side by side 0.84-1.07).  MPK_NEWTON_OVERLAP=0/1 and
-        // MPK_LIST_BLOCKS_PER_CU (1..16) override.
-        bool overlap = kPrec == MPK_PREC_FP64;
-        if (const char* e = std::getenv("MPK_NEWTON_OVERLAP")) overlap = std::atoi(e) != 0;
-        size_t per_cu = overlap ? kSidePerCu : 16;
-        if (const char* e = std::getenv("MPK_LIST_BLOCKS_PER_CU")) {
-            const int v = std::atoi(e);
-            if (v >= 1 && v <= 16) per_cu = (size_t)v;
-        }
-        if (noalloc) {
-            overlap = false;
-            per_cu = 16;
-        }
         auto listed = [&](hipStream_t s) {
             return with_m ? launch_ring_list(plan,
                                              FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals,
@@ -627,10 +631,15 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
                                              2 * sizeof(double), s, list, count, per_cu);
         };
         auto class0 = [&]() {
-            return launch_frame(plan, nullptr,
-                                FrameNewtonLin<kPrec, SV>{sv, k_vals, mfill ? nullptr : m_vals,
-                                                          j_vals, residual, qk, 0},
-                                sizeof(double), 0, st, flags, 0);
+            FrameNewtonLin<kPrec, SV> op{sv, k_vals, mfill ? nullptr : m_vals, j_vals, residual,
+                                         qk, 0};
+            if (!overlap) {
+                op.list_flag = flags;
+                op.list_out = flags + ntiles;
+                op.list_count = flags + 2 * ntiles;
+                op.list_ntiles = ntiles;
+            }
+            return launch_frame(plan, nullptr, op, sizeof(double), 0, st, flags, 0);
         };
         if (!overlap) {
             if (int rc = class0()) return rc;

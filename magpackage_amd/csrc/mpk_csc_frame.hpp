@@ -481,8 +481,29 @@ struct FrameNewtonLin {
     double* __restrict__ res;
     CscQuad qk;
     int acc_bytes_max;
+    // Optional: the list of class-1 tiles, built here instead of by a kernel
+    // of its own.  Block b lists the flagged tiles among [64 b, 64 b + 64)
+    // (ascending), the blocks' ranges appended in arrival order, as
+    // k_tile_list does; list_flag holds the finished classes of every tile
+    // (written by an earlier launch) and *list_count starts at 0.
+    const int32_t* __restrict__ list_flag = nullptr;
+    int32_t* __restrict__ list_out = nullptr;
+    int32_t* __restrict__ list_count = nullptr;
+    int64_t list_ntiles = 0;
     __device__ static Acc zero() { return 0.0; }
-    __device__ void block_init(unsigned char*) const {}
+    __device__ void block_init(unsigned char*) const {
+        if (!list_flag) return;
+        const int64_t t = (int64_t)blockIdx.x * kW + threadIdx.x;
+        if ((int64_t)blockIdx.x * kW >= list_ntiles) return;  // uniform per block
+        const bool on = t < list_ntiles && list_flag[t] != 0;
+        const unsigned long long m = __ballot(on);
+        if (m == 0ull) return;
+        const int lane = threadIdx.x;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(list_count, __popcll(m));
+        base = __shfl(base, 0);
+        if (on) list_out[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)t;
+    }
     __device__ Col col_begin() const { return Col{0.0}; }
     __device__ void col_end(int64_t j, Col& cs) const {
         if (res) res[j] = cs.r;
