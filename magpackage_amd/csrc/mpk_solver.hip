@@ -15,6 +15,7 @@
 // final sum runs in slot order in one block, so results are deterministic.
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 #include "mpk_common.hpp"
 
@@ -25,35 +26,91 @@ constexpr int kB = 256;
 constexpr int kRowLanes = 8;             // lanes cooperating on one CSR row
 constexpr int kMaxPartials = 4096;       // reduction slots (grid-stride kernels)
 
-// y = A_bc x where A is symmetric and stored CSC (== CSR of A).  Fixed rows and
-// columns are removed and replaced by the identity.
+// y = A_bc x where A is symmetric and stored CSC (== CSR of A), for a vector x
+// that vanishes on the fixed DOFs (every CG search direction here does: r, z
+// and p are zero on fixed rows).  Fixed rows are replaced by the identity;
+// fixed columns then contribute A_ij * 0 = 0 and need no test.
+//
+// Segmented form: each wave owns kRows consecutive rows, streams their
+// entries [colptr[r0], colptr[r0 + kRows]) with coalesced loads (64
+// consecutive entries per load, a window's loads and gathers in flight),
+// writes the products v_k x_j to a per-wave LDS window, and each lane then
+// sums its own row's products in entry order (deterministic, independent of
+// the row-length distribution).  Rows longer than the window are summed
+// window by window in the same order.
+// kU loads of 64 entries per wave are issued before their gathers; the LDS
+// window holds 24 products per row (1.5 times the 16 of a Kuhn-mesh edge
+// row), so a group of kRows rows usually takes one window.
+template <int kRows>
+struct SegWin {
+    static constexpr int value = kRows * 24;  // LDS products per wave
+};
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Row sums s_i = sum_k vals[k] x[rowind[k]] over the rows of this wave's
+// groups of kRows rows (grid-stride over the groups), each handed to
+// epi(i, s_i) by the lane that owns row i.  pw: this wave's LDS window.
+template <int kRows, int kU, class Epi>
+__device__ __forceinline__ void seg_rows(const int32_t* __restrict__ colptr,
+                                         const int32_t* __restrict__ rowind,
+                                         const double* __restrict__ vals, int64_t n,
+                                         const double* __restrict__ x, double* pw, Epi&& epi) {
+    constexpr int kWin = SegWin<kRows>::value;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / 64) * kRows;
+    for (int64_t r0 = ((int64_t)blockIdx.x * (kB / 64) + wv) * kRows; r0 < n;
+         r0 += rows_per_grid) {
+        const int64_t rend = r0 + kRows < n ? r0 + kRows : n;
+        const bool own = lane < rend - r0;  // lane owns row r0 + lane
+        const int64_t i = own ? r0 + lane : r0;
+        const int rs = colptr[i];
+        const int re = own ? colptr[i + 1] : rs;
+        const int base = colptr[r0], top = colptr[rend];
+        double s = 0.0;
+        for (int w0 = base; w0 < top; w0 += kWin) {
+            const int w1 = top - w0 < kWin ? top : w0 + kWin;
+            for (int c = w0; c < w1; c += 64 * kU) {
+                int kk[kU], jj[kU];
+                double v[kU], xv[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    kk[u] = c + u * 64 + lane;
+                    const int kc = kk[u] < w1 ? kk[u] : w1 - 1;  // a valid entry
+                    jj[u] = rowind[kc];
+                    v[u] = vals[kc];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) xv[u] = x[jj[u]];
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (kk[u] < w1) pw[kk[u] - w0] = v[u] * xv[u];
+            }
+            wave_lds_sync();
+            const int a = rs > w0 ? rs : w0, b = re < w1 ? re : w1;
+            for (int k = a; k < b; ++k) s += pw[k - w0];
+            wave_lds_sync();  // every lane has read the window before it is reused
+        }
+        if (own) epi(i, s);
+    }
+}
+
+template <int kRows, int kU>
 __global__ void __launch_bounds__(kB)
 k_spmv_dot(const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
            const double* __restrict__ vals, const uint8_t* __restrict__ fixed, int64_t n,
            const double* __restrict__ x, double* __restrict__ y, double* __restrict__ partial) {
+    __shared__ double prod[kB / 64][SegWin<kRows>::value];
     __shared__ double red[kB / 64];
-    const int lane = threadIdx.x % kRowLanes;
-    const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / kRowLanes);
     double acc_dot = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * (kB / kRowLanes) + threadIdx.x / kRowLanes; i < n;
-         i += rows_per_grid) {
-        double s = 0.0;
-        const bool fi = fixed && fixed[i];
-        if (!fi) {
-            for (int k = colptr[i] + lane; k < colptr[i + 1]; k += kRowLanes) {
-                const int j = rowind[k];
-                if (!(fixed && fixed[j])) s += vals[k] * x[j];
-            }
-        }
-        // reduce the row over its kRowLanes lanes (fixed butterfly order)
-#pragma unroll
-        for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kRowLanes);
-        if (fi) s = x[i];
-        if (lane == 0) {
-            y[i] = s;
-            acc_dot += s * x[i];
-        }
-    }
+    seg_rows<kRows, kU>(colptr, rowind, vals, n, x, prod[threadIdx.x >> 6], [&](int64_t i, double s) {
+        if (fixed && fixed[i]) s = x[i];
+        y[i] = s;
+        acc_dot += s * x[i];
+    });
     // block reduction of p.q
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc_dot += __shfl_xor(acc_dot, off, 64);
@@ -66,12 +123,26 @@ k_spmv_dot(const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowin
     }
 }
 
-// Sum of nparts partials in slot order (one block of 256 lanes, fixed tree).
+// Sum of nparts partials in slot order (one block of 256 lanes, fixed tree):
+// lane l adds slots l, l + 256, ... in that order (four loads issued ahead of
+// their additions), then a fixed tree over the lanes.
+__device__ __forceinline__ double lane_sum_fixed(const double* __restrict__ partial, int nparts) {
+    double t = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * kB < nparts; i += 4 * kB) {
+        const double a = partial[i], b = partial[i + kB], c = partial[i + 2 * kB],
+                     d = partial[i + 3 * kB];
+        t += a;
+        t += b;
+        t += c;
+        t += d;
+    }
+    for (; i < nparts; i += kB) t += partial[i];
+    return t;
+}
 __device__ double block_sum_fixed(const double* __restrict__ partial, int nparts) {
     __shared__ double sh[kB];
-    double t = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kB) t += partial[i];
-    sh[threadIdx.x] = t;
+    sh[threadIdx.x] = lane_sum_fixed(partial, nparts);
     __syncthreads();
     for (int s = kB / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
@@ -80,6 +151,24 @@ __device__ double block_sum_fixed(const double* __restrict__ partial, int nparts
     const double r = sh[0];
     __syncthreads();
     return r;
+}
+// Two such sums at once (the same order for each as block_sum_fixed).
+__device__ void block_sum2_fixed(const double* __restrict__ pa, const double* __restrict__ pb,
+                                 int nparts, double* ra, double* rb) {
+    __shared__ double sh[2][kB];
+    sh[0][threadIdx.x] = lane_sum_fixed(pa, nparts);
+    sh[1][threadIdx.x] = lane_sum_fixed(pb, nparts);
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    *ra = sh[0][0];
+    *rb = sh[1][0];
+    __syncthreads();
 }
 
 // scal[0] = rz, scal[1] = alpha, scal[2] = beta, scal[3] = |r|^2
@@ -130,8 +219,8 @@ k_update(int64_t n, const double* __restrict__ scal, const double* __restrict__ 
 __global__ void __launch_bounds__(kB) k_beta(const double* __restrict__ prz,
                                              const double* __restrict__ prr, int nparts,
                                              double* __restrict__ scal) {
-    const double rz_new = block_sum_fixed(prz, nparts);
-    const double rr = block_sum_fixed(prr, nparts);
+    double rz_new, rr;
+    block_sum2_fixed(prz, prr, nparts, &rz_new, &rr);
     if (threadIdx.x == 0) {
         scal[2] = (scal[0] != 0.0) ? rz_new / scal[0] : 0.0;
         scal[0] = rz_new;
@@ -272,9 +361,13 @@ k_update_r(int64_t n, const double* __restrict__ scal, const double* __restrict_
 }
 
 // g = G^T r: node v sums +-r over its edges (ref = dof << 1 | v is the higher end).
+// With phi0 given, also the first damped Jacobi sweep from zero,
+// phi0 = omega D^-1 g (one launch fewer than a kernel of its own).
 __global__ void __launch_bounds__(kB)
 k_gt(int64_t nv, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nref,
-     const double* __restrict__ r, double* __restrict__ g) {
+     const double* __restrict__ r, double* __restrict__ g,
+     const double* __restrict__ dninv = nullptr, double omega = 0.0,
+     double* __restrict__ phi0 = nullptr) {
     for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kB) {
         double s = 0.0;
         for (int k = nptr[v]; k < nptr[v + 1]; ++k) {
@@ -283,14 +376,8 @@ k_gt(int64_t nv, const int32_t* __restrict__ nptr, const int32_t* __restrict__ n
             s += (ref & 1) ? rv : -rv;
         }
         g[v] = s;
+        if (phi0) phi0[v] = omega * dninv[v] * s;
     }
-}
-
-__global__ void __launch_bounds__(kB)
-k_jacobi_first(int64_t nv, const double* __restrict__ dninv, const double* __restrict__ g,
-               double omega, double* __restrict__ phi) {
-    for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kB)
-        phi[v] = omega * dninv[v] * g[v];
 }
 
 // phi_out = phi_in + omega D^-1 (g - Ln phi_in), kRowLanes lanes per row.
@@ -629,6 +716,37 @@ int grid_parts(int64_t n, int per_block) {
     return (int)g;
 }
 
+// Shape of the segmented SpMV: rows per wave and loads in flight, 32 x 4 by
+// default, or one of 16x4, 32x2, 32x8, 64x4 from the environment
+// (MPK_SPMV_SHAPE; a tuning knob: the per-row summation order is the same for
+// every shape).
+int spmv_shape() {
+    const char* e = std::getenv("MPK_SPMV_SHAPE");
+    if (!e) return 0;
+    if (!std::strcmp(e, "16x4")) return 1;
+    if (!std::strcmp(e, "32x2")) return 2;
+    if (!std::strcmp(e, "32x8")) return 3;
+    if (!std::strcmp(e, "64x4")) return 4;
+    return 0;
+}
+int spmv_rows(int shape) { return shape == 1 ? 16 : shape == 4 ? 64 : 32; }
+int spmv_grid(int64_t n, int rows) { return grid_parts(n, (kB / 64) * rows); }
+void launch_spmv_dot(int shape, int grid, hipStream_t st, const int32_t* colptr,
+                     const int32_t* rowind, const double* vals, const uint8_t* fixed, int64_t n,
+                     const double* x, double* y, double* partial) {
+#define MPK_SPMV(R, U)                                                                    \
+    hipLaunchKernelGGL((k_spmv_dot<R, U>), dim3(grid), dim3(kB), 0, st, colptr, rowind, vals, \
+                       fixed, n, x, y, partial)
+    switch (shape) {
+        case 1: MPK_SPMV(16, 4); break;
+        case 2: MPK_SPMV(32, 2); break;
+        case 3: MPK_SPMV(32, 8); break;
+        case 4: MPK_SPMV(64, 4); break;
+        default: MPK_SPMV(32, 4);
+    }
+#undef MPK_SPMV
+}
+
 // Solver stream: a solve runs on a private stream of this thread and device,
 // ordered after the caller's stream by an event at the start and before it by
 // one at the end.  Its iteration batches are captured once into a HIP graph
@@ -767,7 +885,8 @@ int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* va
     double* pc = pb + kMaxPartials;
     double* scal = pc + kMaxPartials;
     const int g_vec = grid_parts(n, kB);
-    const int g_row = grid_parts(n, kB / kRowLanes);
+    const int shape = spmv_shape();
+    const int g_row = spmv_grid(n, spmv_rows(shape));
     hipLaunchKernelGGL(k_setup, dim3(g_vec), dim3(kB), 0, st, colptr, rowind, vals, fixed, n, b,
                        dinv, r, z, p, x, pb, pc);
     hipLaunchKernelGGL(k_setup_scal, dim3(1), dim3(kB), 0, st, pb, pc, g_vec, scal);
@@ -782,8 +901,7 @@ int mpk_pcg_solve(const int32_t* colptr, const int32_t* rowind, const double* va
     while (it < max_iter) {
         const int batch = (max_iter - it) < check_every ? (max_iter - it) : check_every;
         MPK_HIP_TRY(run_batch(ss, batch, check_every, [&]() {
-            hipLaunchKernelGGL(k_spmv_dot, dim3(g_row), dim3(kB), 0, st, colptr, rowind, vals,
-                               fixed, n, p, q, pa);
+            launch_spmv_dot(shape, g_row, st, colptr, rowind, vals, fixed, n, p, q, pa);
             hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, pa, g_row, scal);
             hipLaunchKernelGGL(k_update, dim3(g_vec), dim3(kB), 0, st, n, scal, p, q, dinv, x, r,
                                z, pb, pc);
@@ -836,7 +954,8 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
     double* pc = pb + kMaxPartials;
     double* scal = pc + kMaxPartials;
     const int g_vec = grid_parts(n, kB);
-    const int g_row = grid_parts(n, kB / kRowLanes);
+    const int shape = spmv_shape();
+    const int g_row = spmv_grid(n, spmv_rows(shape));
     const int g_nv = grid_parts(nv, kB);
     const int g_nrow = grid_parts(nv, kB / kRowLanes);
     const int2* en = reinterpret_cast<const int2*>(aux->edge_nodes);
@@ -844,9 +963,7 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
     auto precondition = [&]() {
         if (nv > 0) {
             hipLaunchKernelGGL(k_gt, dim3(g_nv), dim3(kB), 0, st, nv, aux->node_ptr,
-                               aux->node_refs, r, g);
-            hipLaunchKernelGGL(k_jacobi_first, dim3(g_nv), dim3(kB), 0, st, nv, dninv, g,
-                               aux->omega, phi0);
+                               aux->node_refs, r, g, dninv, aux->omega, phi0);
             double* a = phi0;
             double* c = phi1;
             for (int k = 1; k < aux->sweeps; ++k) {
@@ -883,8 +1000,7 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
     while (it < max_iter) {
         const int batch = (max_iter - it) < check_every ? (max_iter - it) : check_every;
         MPK_HIP_TRY(run_batch(ss, batch, check_every, [&]() {
-            hipLaunchKernelGGL(k_spmv_dot, dim3(g_row), dim3(kB), 0, st, colptr, rowind, vals,
-                               fixed, n, p, q, pa);
+            launch_spmv_dot(shape, g_row, st, colptr, rowind, vals, fixed, n, p, q, pa);
             hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kB), 0, st, pa, g_row, scal);
             hipLaunchKernelGGL(k_update_r, dim3(g_vec), dim3(kB), 0, st, n, scal, p, q, x, r, pc);
             precondition();

@@ -186,7 +186,11 @@ class TransientSolver:
         # "aux": Jacobi + gradient-space correction (mpk_pcg_aux_solve);
         # "jacobi": plain Jacobi-PCG
         self.LinearPreconditioner = "aux"
-        self.AuxSweeps = 2
+        # nodal correction p(Ln) = AuxSweeps damped Jacobi sweeps; one sweep is
+        # the diagonal omega Dn^-1, with no Ln product per iteration (983k-tet
+        # E-core, PCG to 1e-10: 8200 iterations per step against 7880 with two
+        # sweeps, 0.99 s against 1.07 s per step)
+        self.AuxSweeps = 1
         self.AuxOmega = 2.0 / 3.0
         self.Precision = "fp64"
         # first Newton iterate of a step: "previous" (x_prev, the reference's
@@ -346,14 +350,18 @@ class TransientSolver:
 
     # ---------------------------------------------------------------- solve
     def solve(self, space_A, matLib, sigmaMap, sourceFunc, timeSteps, fixedDofs_A,
-              init_A=None, warm_start=None, on_step=None):
+              init_A=None, warm_start=None, on_step=None, resume=None):
         """Runs the time steps; returns (solutions [device tensors], infos).
 
         sourceFunc(t) gives the source map (Assembler.expandSourceMap formats
         or an (Ne, 3) device tensor), or an assembled source vector (a 1-D
         device tensor of NumGlobalDofs entries).  warm_start(ctx) may return a
         candidate first iterate for the step; on_step(ctx) is called after
-        each step.
+        each step.  resume (a snapshot(ctx) taken in an on_step hook of an
+        earlier run on the same problem) continues that run after the
+        snapshot's step: timeSteps are then the steps after it, and the
+        previous two solutions, the time and the regularisation are the
+        snapshot's.
         """
         self._setup(space_A, matLib, sigmaMap, fixedDofs_A)
         n = self.n
@@ -364,7 +372,17 @@ class TransientSolver:
         x_prev2 = x_prev.clone()
         results, infos = [], []
         t_now = 0.0
-        for step, dt in enumerate(np.asarray(timeSteps, dtype=np.float64).ravel(), start=1):
+        step0 = 0
+        if resume is not None:
+            if resume["x"].numel() != n:
+                raise ValueError("resume snapshot is for another problem size")
+            x_prev = resume["x"].clone()
+            x_prev2 = resume["x_prev"].clone()
+            t_now = float(resume["t"])
+            step0 = int(resume["step"])
+            self.reg_abs = float(resume["reg_abs"])
+        for step, dt in enumerate(np.asarray(timeSteps, dtype=np.float64).ravel(),
+                                  start=step0 + 1):
             torch.cuda.synchronize()
             w0 = time.perf_counter()
             t_now += float(dt)
@@ -433,6 +451,13 @@ class TransientSolver:
                     print("    residuals/R0 " + " ".join(f"{h / ref:.1e}" for h in info.history) +
                           f" | PCG per solve {info.pcg_per_solve}", file=sys.stderr, flush=True)
         return results, infos
+
+    def snapshot(self, ctx):
+        """State after the step of an on_step ctx, for solve(..., resume=):
+        the step and its time, its solution and the previous one, and the
+        regularisation fixed at the first step."""
+        return {"step": int(ctx["step"]), "t": float(ctx["t"]), "x": ctx["x"].clone(),
+                "x_prev": ctx["x_prev"].clone(), "reg_abs": float(self.reg_abs)}
 
     def solveAV(self, space_A, space_V, matLib, sigmaMap, sourceFunc, timeSteps, fixedDofs_A,
                   fixedDofs_V=None, init_A=None, init_V=None):

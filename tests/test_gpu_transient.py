@@ -85,3 +85,36 @@ def test_divergence_free_source(gpu):
     assert np.linalg.norm(G[:, keep].T @ f) > 0.1 * np.linalg.norm(f)  # the raw load is not
     assert np.linalg.norm(G[:, keep].T @ Fdf) < 1e-8 * np.linalg.norm(f)
     assert np.all(Fdf[fixed] == 0.0)
+
+
+def test_resume_continues_the_run(gpu):
+    """solve(..., resume=snapshot) after step k reproduces steps k+1.. of the
+    uninterrupted run bit for bit: the same Newton and PCG counts and the
+    same solutions."""
+    import torch
+    from magpackage_amd.assembler import Assembler, DofHandler, FunctionSpace, Mesh
+    from magpackage_amd.boundary import find_outer_boundary_dofs
+    from magpackage_amd.transient import TransientSolver
+    ec = ecore_box(10, 10, 8, seed=3)
+    M = Mesh.from_kuhn(ec.mesh)
+    dh = DofHandler(M)
+    space = FunctionSpace("Nedelec", 1)
+    dh.distributeDofs(space)
+    asm = Assembler(M, dh)
+    fixed = find_outer_boundary_dofs(M, dh, space)
+    lib = {1: create_linear(1.0), 2: marrocco(), 3: create_linear(1.0)}
+    js = torch.from_numpy(ec.js_dir).cuda()
+    src = lambda t: js * (3e7 * np.sin(2 * np.pi * 50 * t))
+    steps = [1e-3] * 6
+    full = TransientSolver(asm)
+    sols, infos = full.solve(space, lib, {2: 2e6}, src, steps, fixed)
+    first = TransientSolver(asm)
+    snap = {}
+    first.solve(space, lib, {2: 2e6}, src, steps[:3], fixed,
+                on_step=lambda ctx: snap.update(first.snapshot(ctx)) if ctx["step"] == 3 else None)
+    second = TransientSolver(asm)
+    sols2, infos2 = second.solve(space, lib, {2: 2e6}, src, steps[3:], fixed, resume=snap)
+    assert [i.step for i in infos2] == [4, 5, 6]
+    for a, b, ia, ib in zip(sols[3:], sols2, infos[3:], infos2):
+        assert (ia.newton_iters, ia.pcg_iters) == (ib.newton_iters, ib.pcg_iters)
+        assert torch.equal(a, b)
