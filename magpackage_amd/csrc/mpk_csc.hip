@@ -246,10 +246,11 @@ k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
         eval_nu(bx * bx + by * by + bz * bz, tag, mat, brk, coe, nu, dnu);
         const double2 ds = put_state(s0, s1, e[k], bx, by, bz, nu, dnu, sg[k]);
         // Tile classes for the split Newton set: an element with a D term
-        // (dnu != 0) or an M term (|sigma| > 1e-12), the tests FrameNewton
-        // applies, marks the column tiles of its 6 DOFs.  Every writer stores
-        // the same value, so plain stores need no atomics.
-        if (tile_flag && ((flag_dnu && ds.x != 0.0) || (flag_sigma && fabs(ds.y) > 1e-12))) {
+        // (dnu != 0, the test FrameNewton applies) or an M term (|sigma| >
+        // 1e-12 on the caller's sigma, the test of the sigma-mass op) marks the
+        // column tiles of its 6 DOFs.  Every writer stores the same value, so
+        // plain stores need no atomics.
+        if (tile_flag && ((flag_dnu && ds.x != 0.0) || (flag_sigma && fabs(sg[k]) > 1e-12))) {
             const int dof[6] = {c[k][0].x, c[k][0].y, c[k][1].x, c[k][1].y, c[k][2].x, c[k][2].y};
 #pragma unroll
             for (int i = 0; i < 6; ++i)
@@ -582,15 +583,20 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     if (int rc = elem_state_prepare(plan, P, tags, sol_a, sigma, mat, st, &sv, flags,
                                     with_m || sigma != nullptr, true, scratch_state, &esr))
         return rc;
-    // Schedule, by measurement on the 4M-tet Newton workload: with the FP64
-    // state the listed pass runs best beside the class-0 pass, on the plan's
-    // side stream with a few persistent blocks per CU (0.90 -> 0.84 ms); with
-    // the FP32 state, after it with the full persistent grid (0.81 ms; side by
-    // side 0.84-1.07).  MPK_NEWTON_OVERLAP=0/1 and MPK_LIST_BLOCKS_PER_CU
-    // (1..16) override.
-    bool overlap = kPrec == MPK_PREC_FP64;
+    // Schedule, by measurement on the 4M-tet Newton workload: the listed
+    // tiles run beside the class-0 pass, on the plan's side stream with a few
+    // persistent blocks per CU.  FP64 state: one three-plane pass, 2 blocks
+    // per CU (0.90 -> 0.84 ms at the time; its split into a two-plane pass
+    // and a mass pass measured 0.705-0.708 against 0.699 ms).  FP32 state:
+    // the two-plane pass and the mass pass, 4 blocks per CU (0.657 ms; the
+    // same two passes after the class-0 pass 0.663-0.664 ms, the three-plane
+    // pass after it 0.663-0.665 ms).  MPK_NEWTON_OVERLAP=0/1,
+    // MPK_NEWTON_SPLIT_M=0/1 and MPK_LIST_BLOCKS_PER_CU (1..16) override.
+    bool overlap = true;
     if (const char* e = std::getenv("MPK_NEWTON_OVERLAP")) overlap = std::atoi(e) != 0;
-    size_t per_cu = overlap ? kSidePerCu : 16;
+    bool split_m = kPrec == MPK_PREC_FP32_LOCAL;
+    if (const char* e = std::getenv("MPK_NEWTON_SPLIT_M")) split_m = std::atoi(e) != 0;
+    size_t per_cu = overlap ? (kPrec == MPK_PREC_FP64 ? kSidePerCu : 4) : 16;
     if (const char* e = std::getenv("MPK_LIST_BLOCKS_PER_CU")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 16) per_cu = (size_t)v;
@@ -618,17 +624,26 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     if (split) {
         const int32_t* list = flags + ntiles;
         const int32_t* count = flags + 2 * ntiles;
+        // The listed tiles: K, J and R from two LDS planes (K, D), then M from
+        // a pass of the sigma-mass op over the same list (one plane).  Three
+        // planes in one pass (MPK_NEWTON_SPLIT_M=0) fit 6 blocks per CU; the
+        // two-plane pass fits 9, the mass pass 19 (4M-tet Newton workload,
+        // FP32_LOCAL: 87 us for the three-plane pass).
         auto listed = [&](hipStream_t s) {
-            return with_m ? launch_ring_list(plan,
-                                             FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals,
-                                                                          j_vals, residual, qk,
-                                                                          qm, 0, 0},
-                                             3 * sizeof(double), s, list, count, per_cu)
-                          : launch_ring_list(plan,
-                                             FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr,
-                                                                           j_vals, residual, qk,
-                                                                           qm, 0, 0},
-                                             2 * sizeof(double), s, list, count, per_cu);
+            if (with_m && !split_m)
+                return launch_ring_list(plan,
+                                        FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals, j_vals,
+                                                                     residual, qk, qm, 0, 0},
+                                        3 * sizeof(double), s, list, count, per_cu);
+            if (int rc = launch_ring_list(plan,
+                                          FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr,
+                                                                        j_vals, residual, qk, qm,
+                                                                        0, 0},
+                                          2 * sizeof(double), s, list, count, per_cu))
+                return rc;
+            if (!with_m) return (int)MPK_OK;
+            return launch_ring_list(plan, FrameMass<kPrec>{sigma, m_vals, qm, 0}, sizeof(double),
+                                    s, list, count, per_cu);
         };
         auto class0 = [&]() {
             FrameNewtonLin<kPrec, SV> op{sv, k_vals, mfill ? nullptr : m_vals, j_vals, residual,

@@ -8,11 +8,13 @@
 // (symmetric, structural-pattern) CSC values and a fixed-DOF mask, so the pattern is
 // never rebuilt.
 //
-// Jacobi-preconditioned conjugate gradients with every scalar kept on the device:
-// one iteration is 4 kernels (SpMV+dot, alpha, update+dot, beta+direction) and
-// needs no host round trip; the host checks convergence every `check_every`
-// iterations.  Dot products reduce per block into a fixed slot array and the
-// final sum runs in slot order in one block, so results are deterministic.
+// Preconditioned conjugate gradients with every scalar kept on the device: one
+// Jacobi-PCG iteration is 5 kernels (SpMV + p.q, alpha, update + r.z and r.r,
+// beta, direction), the auxiliary-space one adds G^T r and the preconditioned
+// residual; no host round trip, the host checks convergence every
+// `check_every` iterations.  Dot products reduce per block into a fixed slot
+// array and the final sum runs in slot order in one block, so results are
+// deterministic.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -39,11 +41,11 @@ constexpr int kMaxPartials = 4096;       // reduction slots (grid-stride kernels
 // the row-length distribution).  Rows longer than the window are summed
 // window by window in the same order.
 // kU loads of 64 entries per wave are issued before their gathers; the LDS
-// window holds 24 products per row (1.5 times the 16 of a Kuhn-mesh edge
-// row), so a group of kRows rows usually takes one window.
-template <int kRows>
+// window holds kPerRow products per row (the rows of a Kuhn-mesh edge space
+// hold about 16), a group of kRows rows one or two windows.
+template <int kRows, int kPerRow>
 struct SegWin {
-    static constexpr int value = kRows * 24;  // LDS products per wave
+    static constexpr int value = kRows * kPerRow;  // LDS products per wave
 };
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -54,12 +56,12 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Row sums s_i = sum_k vals[k] x[rowind[k]] over the rows of this wave's
 // groups of kRows rows (grid-stride over the groups), each handed to
 // epi(i, s_i) by the lane that owns row i.  pw: this wave's LDS window.
-template <int kRows, int kU, class Epi>
+template <int kRows, int kU, int kPerRow, class Epi>
 __device__ __forceinline__ void seg_rows(const int32_t* __restrict__ colptr,
                                          const int32_t* __restrict__ rowind,
                                          const double* __restrict__ vals, int64_t n,
                                          const double* __restrict__ x, double* pw, Epi&& epi) {
-    constexpr int kWin = SegWin<kRows>::value;
+    constexpr int kWin = SegWin<kRows, kPerRow>::value;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / 64) * kRows;
     for (int64_t r0 = ((int64_t)blockIdx.x * (kB / 64) + wv) * kRows; r0 < n;
@@ -98,15 +100,16 @@ __device__ __forceinline__ void seg_rows(const int32_t* __restrict__ colptr,
     }
 }
 
-template <int kRows, int kU>
+template <int kRows, int kU, int kPerRow>
 __global__ void __launch_bounds__(kB)
 k_spmv_dot(const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
            const double* __restrict__ vals, const uint8_t* __restrict__ fixed, int64_t n,
            const double* __restrict__ x, double* __restrict__ y, double* __restrict__ partial) {
-    __shared__ double prod[kB / 64][SegWin<kRows>::value];
+    __shared__ double prod[kB / 64][SegWin<kRows, kPerRow>::value];
     __shared__ double red[kB / 64];
     double acc_dot = 0.0;
-    seg_rows<kRows, kU>(colptr, rowind, vals, n, x, prod[threadIdx.x >> 6], [&](int64_t i, double s) {
+    seg_rows<kRows, kU, kPerRow>(colptr, rowind, vals, n, x, prod[threadIdx.x >> 6],
+                                 [&](int64_t i, double s) {
         if (fixed && fixed[i]) s = x[i];
         y[i] = s;
         acc_dot += s * x[i];
@@ -360,23 +363,31 @@ k_update_r(int64_t n, const double* __restrict__ scal, const double* __restrict_
     }
 }
 
-// g = G^T r: node v sums +-r over its edges (ref = dof << 1 | v is the higher end).
-// With phi0 given, also the first damped Jacobi sweep from zero,
-// phi0 = omega D^-1 g (one launch fewer than a kernel of its own).
+// g = G^T r: node v sums +-r over its edges (ref = dof << 1 | v is the higher
+// end), kRowLanes lanes per node (fixed butterfly order).  With phi0 given,
+// also the first damped Jacobi sweep from zero, phi0 = omega D^-1 g (one
+// launch fewer than a kernel of its own).
 __global__ void __launch_bounds__(kB)
 k_gt(int64_t nv, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nref,
      const double* __restrict__ r, double* __restrict__ g,
      const double* __restrict__ dninv = nullptr, double omega = 0.0,
      double* __restrict__ phi0 = nullptr) {
-    for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < nv; v += (int64_t)gridDim.x * kB) {
+    const int lane = threadIdx.x % kRowLanes;
+    const int64_t rows_per_grid = (int64_t)gridDim.x * (kB / kRowLanes);
+    for (int64_t v = (int64_t)blockIdx.x * (kB / kRowLanes) + threadIdx.x / kRowLanes; v < nv;
+         v += rows_per_grid) {
         double s = 0.0;
-        for (int k = nptr[v]; k < nptr[v + 1]; ++k) {
+        for (int k = nptr[v] + lane; k < nptr[v + 1]; k += kRowLanes) {
             const int ref = nref[k];
             const double rv = r[ref >> 1];
             s += (ref & 1) ? rv : -rv;
         }
-        g[v] = s;
-        if (phi0) phi0[v] = omega * dninv[v] * s;
+#pragma unroll
+        for (int off = kRowLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kRowLanes);
+        if (lane == 0) {
+            g[v] = s;
+            if (phi0) phi0[v] = omega * dninv[v] * s;
+        }
     }
 }
 
@@ -716,10 +727,13 @@ int grid_parts(int64_t n, int per_block) {
     return (int)g;
 }
 
-// Shape of the segmented SpMV: rows per wave and loads in flight, 32 x 4 by
-// default, or one of 16x4, 32x2, 32x8, 64x4 from the environment
-// (MPK_SPMV_SHAPE; a tuning knob: the per-row summation order is the same for
-// every shape).
+// Shape of the segmented SpMV: rows per wave, loads in flight and LDS
+// products per row.  Default 32 x 4 x 16: a 16 KB block allows 8 waves per
+// SIMD, and a group longer than 512 entries takes a second window (983k-tet
+// E-core Newton systems: 60 us per product, against 64 us with 20 products
+// per row and 68 us with 24).  MPK_SPMV_SHAPE picks 16x4, 32x2, 32x8, 64x4
+// (24 products per row), 32x4w20 or 32x4w24 instead (a tuning knob: the
+// per-row summation order is the same for every shape).
 int spmv_shape() {
     const char* e = std::getenv("MPK_SPMV_SHAPE");
     if (!e) return 0;
@@ -727,6 +741,8 @@ int spmv_shape() {
     if (!std::strcmp(e, "32x2")) return 2;
     if (!std::strcmp(e, "32x8")) return 3;
     if (!std::strcmp(e, "64x4")) return 4;
+    if (!std::strcmp(e, "32x4w20")) return 5;
+    if (!std::strcmp(e, "32x4w24")) return 6;
     return 0;
 }
 int spmv_rows(int shape) { return shape == 1 ? 16 : shape == 4 ? 64 : 32; }
@@ -734,15 +750,17 @@ int spmv_grid(int64_t n, int rows) { return grid_parts(n, (kB / 64) * rows); }
 void launch_spmv_dot(int shape, int grid, hipStream_t st, const int32_t* colptr,
                      const int32_t* rowind, const double* vals, const uint8_t* fixed, int64_t n,
                      const double* x, double* y, double* partial) {
-#define MPK_SPMV(R, U)                                                                    \
-    hipLaunchKernelGGL((k_spmv_dot<R, U>), dim3(grid), dim3(kB), 0, st, colptr, rowind, vals, \
-                       fixed, n, x, y, partial)
+#define MPK_SPMV(R, U, W)                                                                    \
+    hipLaunchKernelGGL((k_spmv_dot<R, U, W>), dim3(grid), dim3(kB), 0, st, colptr, rowind,       \
+                       vals, fixed, n, x, y, partial)
     switch (shape) {
-        case 1: MPK_SPMV(16, 4); break;
-        case 2: MPK_SPMV(32, 2); break;
-        case 3: MPK_SPMV(32, 8); break;
-        case 4: MPK_SPMV(64, 4); break;
-        default: MPK_SPMV(32, 4);
+        case 1: MPK_SPMV(16, 4, 24); break;
+        case 2: MPK_SPMV(32, 2, 24); break;
+        case 3: MPK_SPMV(32, 8, 24); break;
+        case 4: MPK_SPMV(64, 4, 24); break;
+        case 5: MPK_SPMV(32, 4, 20); break;
+        case 6: MPK_SPMV(32, 4, 24); break;
+        default: MPK_SPMV(32, 4, 16);
     }
 #undef MPK_SPMV
 }
@@ -962,7 +980,7 @@ int mpk_pcg_aux_solve(const int32_t* colptr, const int32_t* rowind, const double
     // z = M^-1 r; partial r.z into pb
     auto precondition = [&]() {
         if (nv > 0) {
-            hipLaunchKernelGGL(k_gt, dim3(g_nv), dim3(kB), 0, st, nv, aux->node_ptr,
+            hipLaunchKernelGGL(k_gt, dim3(g_nrow), dim3(kB), 0, st, nv, aux->node_ptr,
                                aux->node_refs, r, g, dninv, aux->omega, phi0);
             double* a = phi0;
             double* c = phi1;
