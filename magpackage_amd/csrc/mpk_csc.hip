@@ -93,36 +93,6 @@ __device__ inline double2 put_state(float4* s0, float2* s1, int64_t e, double bx
     return make_double2(v.x, v.y);
 }
 
-// Zeroes n doubles with streaming (non-temporal) stores: kFillPairs per
-// thread, coalesced 64-lane runs of 16-byte stores.  With flag given, thread i
-// of the grid also clears flag[i] (i < ntiles) and thread 0 the count (the
-// work of k_clear_tiles, so the Newton set needs no launch of its own for it).
-constexpr int kFillBlock = 256;
-constexpr int kFillPairs = 2;  // 16-byte stores per thread
-__global__ void __launch_bounds__(kFillBlock)
-k_fill_zero(double* __restrict__ p, int64_t n, int32_t* __restrict__ flag = nullptr,
-            int64_t ntiles = 0, int32_t* __restrict__ count = nullptr) {
-    if (flag) {
-        const int64_t t = (int64_t)blockIdx.x * kFillBlock + threadIdx.x;
-        if (t < ntiles) flag[t] = 0;
-        if (t == 0) *count = 0;
-    }
-    // pairs of doubles (16-byte stores) from the first 16-byte aligned entry
-    const int64_t a = (int64_t)((reinterpret_cast<uintptr_t>(p) >> 3) & 1u);
-    const int64_t npair = (n - a) >> 1;
-    double2* p2 = reinterpret_cast<double2*>(p + a);
-    const int64_t b = (int64_t)blockIdx.x * (kFillPairs * kFillBlock) + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kFillPairs; ++k) {
-        const int64_t q = b + (int64_t)k * kFillBlock;
-        if (q < npair) st_out(p2 + q, make_double2(0.0, 0.0));
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (a && n > 0) st_out(p, 0.0);
-        if (n > a && ((n - a) & 1)) st_out(p + n - 1, 0.0);
-    }
-}
-
 // Clears the tile class flags and the class-1 tile count.
 __global__ void k_clear_tiles(int32_t* __restrict__ flag, int64_t ntiles,
                               int32_t* __restrict__ count) {
@@ -131,17 +101,17 @@ __global__ void k_clear_tiles(int32_t* __restrict__ flag, int64_t ntiles,
     if (i == 0) *count = 0;
 }
 
-// Lists the class-1 tiles: ascending within each block of 256 tiles, the
-// blocks' ranges appended in arrival order (the tiles are independent, so the
-// list order changes no result).
+// Lists the class-1 tiles (flag == epoch): ascending within each block of 256
+// tiles, the blocks' ranges appended in arrival order (the tiles are
+// independent, so the list order changes no result).
 constexpr int kListBlock = 256;
 __global__ void __launch_bounds__(kListBlock)
 k_tile_list(const int32_t* __restrict__ flag, int64_t ntiles, int32_t* __restrict__ list,
-            int32_t* __restrict__ count) {
+            int32_t* __restrict__ count, int32_t epoch) {
     __shared__ int wsum[kListBlock / 64];
     __shared__ int base;
     const int64_t t = (int64_t)blockIdx.x * kListBlock + threadIdx.x;
-    const bool on = t < ntiles && flag[t] != 0;
+    const bool on = t < ntiles && flag[t] == epoch;
     const unsigned long long m = __ballot(on);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int before = __popcll(m & ((1ull << lane) - 1ull));
@@ -172,8 +142,27 @@ k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
              const double* __restrict__ sol,
              const int32_t* __restrict__ tags, const double* __restrict__ sigma, MaterialView mat,
              int ntags, int nbrk, int ncoe, S0* __restrict__ s0, S1* __restrict__ s1,
-             int32_t* __restrict__ tile_flag, int flag_sigma, int flag_dnu) {
+             int32_t* __restrict__ tile_flag, int flag_sigma, int flag_dnu, int32_t epoch,
+             int32_t* __restrict__ count, double* __restrict__ mz, int64_t mz_n) {
     __shared__ double tab[kMatLds];
+    // Zeroes mz[0, mz_n) (M of the Newton set; its tiles with an M term are
+    // rewritten by the pass that runs them) with streaming 16-byte stores,
+    // grid-stride over the pairs: fire-and-forget stores issued while this
+    // gather-bound kernel waits on its loads.
+    if (mz) {
+        const int64_t a = (int64_t)((reinterpret_cast<uintptr_t>(mz) >> 3) & 1u);
+        const int64_t npair = (mz_n - a) >> 1;
+        double2* p2 = reinterpret_cast<double2*>(mz + a);
+        const int64_t nth = (int64_t)gridDim.x * kStateBlock;
+        for (int64_t q = (int64_t)blockIdx.x * kStateBlock + threadIdx.x; q < npair; q += nth)
+            st_out(p2 + q, make_double2(0.0, 0.0));
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (a && mz_n > 0) st_out(mz, 0.0);
+            if (mz_n > a && ((mz_n - a) & 1)) st_out(mz + mz_n - 1, 0.0);
+        }
+    }
+    // the class-1 tile count starts at 0 (no kernel reads it before this one ends)
+    if (count && blockIdx.x == 0 && threadIdx.x == 0) *count = 0;
     const double* brk = mat.brk;
     const double* coe = mat.coe;
     if (nbrk + ncoe <= kMatLds) {
@@ -248,13 +237,13 @@ k_elem_state(const int4* __restrict__ T, const int32_t* __restrict__ cell,
         // Tile classes for the split Newton set: an element with a D term
         // (dnu != 0, the test FrameNewton applies) or an M term (|sigma| >
         // 1e-12 on the caller's sigma, the test of the sigma-mass op) marks the
-        // column tiles of its 6 DOFs.  Every writer stores the same value, so
-        // plain stores need no atomics.
+        // column tiles of its 6 DOFs with the call's epoch.  Every writer stores
+        // the same value, so plain stores need no atomics.
         if (tile_flag && ((flag_dnu && ds.x != 0.0) || (flag_sigma && fabs(sg[k]) > 1e-12))) {
             const int dof[6] = {c[k][0].x, c[k][0].y, c[k][1].x, c[k][1].y, c[k][2].x, c[k][2].y};
 #pragma unroll
             for (int i = 0; i < 6; ++i)
-                if (dof[i] >= 0) tile_flag[dof[i] / kTileCols] = 1;
+                if (dof[i] >= 0) tile_flag[dof[i] / kTileCols] = epoch;
         }
     }
 }
@@ -273,6 +262,9 @@ struct ElemStateRun {
     typename SV::S1* s1 = nullptr;
     int32_t* flag = nullptr;
     int flag_sigma = 0, flag_dnu = 0;
+    int32_t epoch = 1;
+    double* mz = nullptr;  // zeroed by the state kernel (the Newton set's M)
+    int64_t mz_n = 0;
     int64_t ntiles = 0;
     hipError_t clear(hipStream_t st) const {
         if (flag)
@@ -286,13 +278,14 @@ struct ElemStateRun {
                                dim3(grid_for(e - b, kStateBlock * kStateElems)),
                                dim3(kStateBlock), 0, st, plan->T, plan->cell, plan->P4, b, e, sol,
                                tags, sigma, mv, ntags, nbrk, ncoe, s0, s1, flag, flag_sigma,
-                               flag_dnu);
+                               flag_dnu, epoch, flag ? flag + 2 * ntiles : nullptr,
+                               b == 0 ? mz : nullptr, mz_n);
         return hipGetLastError();
     }
     hipError_t list(hipStream_t st) const {
         if (flag)
             hipLaunchKernelGGL(k_tile_list, dim3(grid_for(ntiles, kListBlock)), dim3(kListBlock),
-                               0, st, flag, ntiles, flag + ntiles, flag + 2 * ntiles);
+                               0, st, flag, ntiles, flag + ntiles, flag + 2 * ntiles, epoch);
         return hipGetLastError();
     }
 };
@@ -418,27 +411,28 @@ bool ring_driver(const mpk_plan* plan) {
 // Launches of the ring kernels for a gather batch chosen at run time.
 template <class Op, bool kC>
 void launch_batch_k(int kb, unsigned grid, size_t lds, hipStream_t st, const TileCtx& tc,
-                    const int2* ends, const Op& op, const int32_t* sel, int32_t want) {
+                    const int2* ends, const Op& op, const int32_t* sel, int32_t want,
+                    int32_t epoch) {
     switch (kb) {
         case 2:
             hipLaunchKernelGGL((k_csc_ring_batch<Op, 2, kC>), dim3(grid), dim3(kW), lds, st, tc,
-                               ends, op, sel, want);
+                               ends, op, sel, want, epoch);
             break;
         case 4:
             hipLaunchKernelGGL((k_csc_ring_batch<Op, 4, kC>), dim3(grid), dim3(kW), lds, st, tc,
-                               ends, op, sel, want);
+                               ends, op, sel, want, epoch);
             break;
         case 6:
             hipLaunchKernelGGL((k_csc_ring_batch<Op, 6, kC>), dim3(grid), dim3(kW), lds, st, tc,
-                               ends, op, sel, want);
+                               ends, op, sel, want, epoch);
             break;
         case 8:
             hipLaunchKernelGGL((k_csc_ring_batch<Op, 8, kC>), dim3(grid), dim3(kW), lds, st, tc,
-                               ends, op, sel, want);
+                               ends, op, sel, want, epoch);
             break;
         default:
             hipLaunchKernelGGL((k_csc_ring_batch<Op, 3, kC>), dim3(grid), dim3(kW), lds, st, tc,
-                               ends, op, sel, want);
+                               ends, op, sel, want, epoch);
     }
 }
 template <class Op, bool kC>
@@ -504,7 +498,8 @@ int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t s
 // sel / want: ring driver only, run the tiles whose class flag equals want.
 template <class Op>
 int launch_frame(const mpk_plan* plan, const double* P, Op op, size_t acc_size, size_t extra,
-                 hipStream_t st, const int32_t* sel = nullptr, int32_t want = 0) {
+                 hipStream_t st, const int32_t* sel = nullptr, int32_t want = 0,
+                 int32_t epoch = 1) {
     if (plan->n == 0) return MPK_OK;
     size_t lds = 0;
     if (int rc = lds_bytes(plan, op, acc_size, extra, &lds)) return rc;
@@ -519,10 +514,10 @@ int launch_frame(const mpk_plan* plan, const double* P, Op op, size_t acc_size, 
         const TileCtx tc = tile_ctx(plan);
         if (plan->ring8)
             launch_batch_k<Op, true>(ring_batch<Op>(), grid, lds, st, tc, plan->col_ends, op, sel,
-                                     want);
+                                     want, epoch);
         else
             launch_batch_k<Op, false>(ring_batch<Op>(), grid, lds, st, tc, plan->col_ends, op,
-                                      sel, want);
+                                      sel, want, epoch);
     } else {
         MPK_REQUIRE(!sel, MPK_ERR_ARGS, "tile selection needs the ring driver");
         hipLaunchKernelGGL(k_csc_frame<Op>, dim3(grid), dim3(kW), lds, st, tile_ctx(plan), op);
@@ -554,24 +549,39 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     const bool noalloc = scratch_state != nullptr;
     int32_t* flags = nullptr;
     const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
+    // Tile classes: the element state writes this call's epoch into the flag
+    // of every class-1 tile.  The plan's flags are cleared once, when they are
+    // allocated, and each call takes the next epoch (cleared again only when
+    // the epoch would wrap); the allocation-free entry point clears its
+    // scratch flags on every call and uses epoch 1.
+    int32_t epoch = 1;
+    bool clear_flags = false;
     if (split) {
         if (noalloc) {
             flags = scratch_flags;
+            clear_flags = true;
         } else {
-            if (!plan->buf_tile_flag.p)
+            if (!plan->buf_tile_flag.p) {
                 MPK_HIP_TRY(
                     plan->buf_tile_flag.alloc((size_t)(2 * ntiles + 1) * sizeof(int32_t)));
+                plan->tile_epoch = 0;
+                clear_flags = true;
+            }
+            if (plan->tile_epoch >= 0x7ffffff0) {
+                plan->tile_epoch = 0;
+                clear_flags = true;
+            }
+            epoch = ++plan->tile_epoch;
             flags = plan->buf_tile_flag.as<int32_t>();
         }
     }
-    // M is zero on the tiles no M term reaches.  A streaming fill writes
-    // zeros over the whole M array before the element state; the class-0
-    // pass then stores K and J only, and the listed pass writes its tiles' M
-    // (4M-tet Newton workload: 0.738-0.747 -> 0.720 ms FP32_LOCAL; run on
-    // the caller's stream rather than beside the element state on a side
-    // stream, which needed fork and join events: 0.691 -> 0.680 ms).
-    // MPK_NEWTON_MFILL=0 stores the zeros from the class-0 pass instead; the
-    // allocation-free entry point always does.
+    // M is zero on the tiles no M term reaches.  The element-state kernel
+    // zeroes the whole M array with streaming stores while it waits on its
+    // gathers; the class-0 pass then stores K and J only, and the listed
+    // passes rewrite their tiles' M (4M-tet Newton workload, FP32_LOCAL: a
+    // fill kernel of its own before the element state, 104 + 111 us, against
+    // 208 us for the two together).  MPK_NEWTON_MFILL=0 stores the zeros from
+    // the class-0 pass instead; the allocation-free entry point always does.
     bool mfill = split && with_m && !noalloc && plan->nnz > 0;
     if (const char* e = std::getenv("MPK_NEWTON_MFILL")) mfill = mfill && std::atoi(e) != 0;
     // The classes depend on sigma and dnu, not on which outputs are requested
@@ -583,6 +593,17 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     if (int rc = elem_state_prepare(plan, P, tags, sol_a, sigma, mat, st, &sv, flags,
                                     with_m || sigma != nullptr, true, scratch_state, &esr))
         return rc;
+    esr.epoch = epoch;
+    // without elements there is no element-state launch to zero M or reset
+    // the count
+    if (split && !esr.plan) {
+        clear_flags = true;
+        mfill = false;
+    }
+    if (mfill) {
+        esr.mz = m_vals;
+        esr.mz_n = plan->nnz;
+    }
     // Schedule, by measurement on the 4M-tet Newton workload: the listed
     // tiles run beside the class-0 pass, on the plan's side stream with a few
     // persistent blocks per CU.  FP64 state: one three-plane pass, 2 blocks
@@ -605,21 +626,19 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
         overlap = false;
         per_cu = 16;
     }
-    if (mfill) {
-        // the fill also clears the tile classes (flags is set: mfill needs split)
-        const unsigned g1 = grid_for(plan->nnz, 2 * kFillPairs * kFillBlock);
-        const unsigned g2 = grid_for(ntiles, kFillBlock);
-        hipLaunchKernelGGL(k_fill_zero, dim3(g1 > g2 ? g1 : g2), dim3(kFillBlock), 0, st, m_vals,
-                           plan->nnz, flags, ntiles, flags + 2 * ntiles);
+    if (clear_flags) {
+        hipLaunchKernelGGL(k_clear_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, st, flags,
+                           ntiles, flags + 2 * ntiles);
         MPK_HIP_TRY(hipGetLastError());
     }
-    if (esr.plan) {
-        if (!mfill) MPK_HIP_TRY(esr.clear(st));
-        MPK_HIP_TRY(esr.run(0, plan->ne, st));
-        // The listed pass needs its list before it starts: beside the class-0
-        // pass, from a kernel of its own; after it, the class-0 pass's first
-        // blocks build it (one launch fewer: 0.678-0.681 -> 0.675-0.678 ms).
-        if (overlap) MPK_HIP_TRY(esr.list(st));
+    if (esr.plan) MPK_HIP_TRY(esr.run(0, plan->ne, st));
+    // The listed passes need their list before they start: beside the class-0
+    // pass, from a kernel of its own; after it, the class-0 pass's first
+    // blocks build it (one launch fewer).
+    if (split && overlap) {
+        hipLaunchKernelGGL(k_tile_list, dim3(grid_for(ntiles, kListBlock)), dim3(kListBlock), 0,
+                           st, flags, ntiles, flags + ntiles, flags + 2 * ntiles, epoch);
+        MPK_HIP_TRY(hipGetLastError());
     }
     if (split) {
         const int32_t* list = flags + ntiles;
@@ -653,8 +672,9 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
                 op.list_out = flags + ntiles;
                 op.list_count = flags + 2 * ntiles;
                 op.list_ntiles = ntiles;
+                op.list_epoch = epoch;
             }
-            return launch_frame(plan, nullptr, op, sizeof(double), 0, st, flags, 0);
+            return launch_frame(plan, nullptr, op, sizeof(double), 0, st, flags, 0, epoch);
         };
         if (!overlap) {
             if (int rc = class0()) return rc;

@@ -482,20 +482,21 @@ struct FrameNewtonLin {
     CscQuad qk;
     int acc_bytes_max;
     // Optional: the list of class-1 tiles, built here instead of by a kernel
-    // of its own.  Block b lists the flagged tiles among [64 b, 64 b + 64)
-    // (ascending), the blocks' ranges appended in arrival order, as
-    // k_tile_list does; list_flag holds the finished classes of every tile
-    // (written by an earlier launch) and *list_count starts at 0.
+    // of its own.  Block b lists the tiles among [64 b, 64 b + 64) whose flag
+    // holds list_epoch (ascending), the blocks' ranges appended in arrival
+    // order, as k_tile_list does; list_flag holds the finished classes of
+    // every tile (written by an earlier launch) and *list_count starts at 0.
     const int32_t* __restrict__ list_flag = nullptr;
     int32_t* __restrict__ list_out = nullptr;
     int32_t* __restrict__ list_count = nullptr;
     int64_t list_ntiles = 0;
+    int32_t list_epoch = 1;
     __device__ static Acc zero() { return 0.0; }
     __device__ void block_init(unsigned char*) const {
         if (!list_flag) return;
         const int64_t t = (int64_t)blockIdx.x * kW + threadIdx.x;
         if ((int64_t)blockIdx.x * kW >= list_ntiles) return;  // uniform per block
-        const bool on = t < list_ntiles && list_flag[t] != 0;
+        const bool on = t < list_ntiles && list_flag[t] == list_epoch;
         const unsigned long long m = __ballot(on);
         if (m == 0ull) return;
         const int lane = threadIdx.x;
@@ -639,7 +640,7 @@ struct RingIdx {
 template <bool kC>
 __device__ __forceinline__ RingIdx ring_load(const TileCtx& tc, const int2* __restrict__ ends,
                                              int64_t tile_id, const int32_t* __restrict__ sel,
-                                             int32_t want) {
+                                             int32_t want, int32_t epoch) {
     const int64_t j0 = tile_id * kW;
     const int64_t jend = j0 + kW < tc.n ? j0 + kW : tc.n;
     const int64_t j = j0 + threadIdx.x;
@@ -648,7 +649,8 @@ __device__ __forceinline__ RingIdx ring_load(const TileCtx& tc, const int2* __re
     ix.base = tc.colptr[j0];
     ix.top = tc.colptr[jend];
     ix.fbase = tc.frame_base[tile_id];
-    ix.cls = sel ? sel[tile_id] : want;
+    // class 1: the tile's flag holds this call's epoch
+    ix.cls = sel ? (sel[tile_id] == epoch ? 1 : 0) : want;
     if constexpr (kC) {
         const uint4 h = tc.col_head[jc];
         ix.head = h;
@@ -807,8 +809,8 @@ template <class Op, int kB, bool kC>
 __device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restrict__ ends, Op& op,
                                           unsigned char* smem, int64_t tile_id,
                                           const int32_t* __restrict__ sel = nullptr,
-                                          int32_t want = 0) {
-    const RingIdx ix = ring_load<kC>(tc, ends, tile_id, sel, want);
+                                          int32_t want = 0, int32_t epoch = 1) {
+    const RingIdx ix = ring_load<kC>(tc, ends, tile_id, sel, want, epoch);
     ring_run<Op, kB, kC>(tc, op, smem, tile_id, ix, want);
 }
 
@@ -817,9 +819,10 @@ __device__ __forceinline__ void ring_tile(const TileCtx& tc, const int2* __restr
 template <class Op, int kB, bool kC>
 __global__ void __launch_bounds__(kW) k_csc_ring_batch(TileCtx tc, const int2* __restrict__ ends,
                                                        Op op, const int32_t* __restrict__ sel,
-                                                       int32_t want) {
+                                                       int32_t want, int32_t epoch) {
     extern __shared__ __align__(16) unsigned char smem[];
-    ring_tile<Op, kB, kC>(tc, ends, op, smem, xcd_tile(blockIdx.x, gridDim.x), sel, want);
+    ring_tile<Op, kB, kC>(tc, ends, op, smem, xcd_tile(blockIdx.x, gridDim.x), sel, want,
+                          epoch);
 }
 
 // Tiles from a device list (persistent grid): block b runs on XCD b % 8; the

@@ -30,9 +30,12 @@ struct mpk_plan {
     mutable mpk::DevBuf buf_state;
     // Tile classes of the fused Newton set, written with the element state,
     // [2 ntiles + 1] int32 (ntiles = column tiles of kTileCols columns):
-    // flag per tile (1 if an element with an M or D term touches the tile),
-    // the list of flagged tiles, their count.  Allocated on first use.
+    // flag per tile (the call's epoch if an element with an M or D term
+    // touches the tile), the list of flagged tiles, their count.  Allocated
+    // (and cleared) on first use; each call takes the next epoch, so the
+    // flags of earlier calls need no clearing.
     mutable mpk::DevBuf buf_tile_flag;
+    mutable int32_t tile_epoch = 0;
     mutable int num_cus = 0;  // compute units of the plan's device (queried on first use)
     // side stream of the fused Newton set (its two tile classes run side by side)
     mutable mpk::SideStream side;
