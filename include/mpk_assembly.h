@@ -275,17 +275,6 @@ int mpk_csc_curl_curl_mass_complex(const mpk_plan* plan, const double* P, const 
                                    const mpk_quadrature* quad_k, const mpk_quadrature* quad_m,
                                    int precision, double* vals_c128, void* stream);
 
-/* mpk_csc_curl_curl_mass_complex with nu and sigma interleaved per element:
- * nu_sigma [2*ne] device, (nu_e, sigma_e) pairs.  Same values (bit for bit)
- * as the two-array call in FP64 / FP32_LOCAL mode; one 16-byte gather per
- * element visit instead of two 8-byte ones.  No counterpart in the reference
- * (its MEX takes the two arrays separately). */
-int mpk_csc_curl_curl_mass_complex_pairs(const mpk_plan* plan, const double* P,
-                                         const double* nu_sigma, double omega,
-                                         const mpk_quadrature* quad_k,
-                                         const mpk_quadrature* quad_m, int precision,
-                                         double* vals_c128, void* stream);
-
 /* Newton Jacobian J and residual R (src/mex/assemble_jacobian_kernel_mex.cpp
  * + sparse() of J and of R, src/Assembler.m:199-216).  tags [ne] device int32
  * (1-based); sol_a [ndofs] device.  j_vals [nnz] (ignored when calc_j == 0);
