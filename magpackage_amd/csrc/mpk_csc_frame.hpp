@@ -630,6 +630,22 @@ __global__ void __launch_bounds__(kW) k_csc_frame(TileCtx tc, Op op) {
 // kC selects the record format: compact (8-byte records + a 16-byte column
 // head; positions of the shared face and orientation bits carried from one
 // incidence to the next) or the 16-byte ring records.
+// Streaming loads (non-temporal hint) of the compact ring records and column
+// heads: each is read once per pass, and the hint keeps them from displacing
+// the gathered node and element data in L2.  Same bits as a plain load.
+// 128^3-box K + jwM: 1.313 -> 1.308 ms (FP64), 1.248 -> 1.238 ms (FP32-local);
+// 4M-tet Newton set 0.665-0.671 -> 0.656-0.657 ms, on one box.
+__device__ __forceinline__ uint2 ld_stream(const uint2* p) {
+    const unsigned long long v =
+        __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(p));
+    return make_uint2((unsigned)v, (unsigned)(v >> 32));
+}
+__device__ __forceinline__ uint4 ld_stream(const uint4* p) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 struct RingIdx {
     int cbase, cnt;  // column's first value index, incidences
     int lo, hi;      // column edge's endpoints
@@ -652,7 +668,7 @@ __device__ __forceinline__ RingIdx ring_load(const TileCtx& tc, const int2* __re
     // class 1: the tile's flag holds this call's epoch
     ix.cls = sel ? (sel[tile_id] == epoch ? 1 : 0) : want;
     if constexpr (kC) {
-        const uint4 h = tc.col_head[jc];
+        const uint4 h = ld_stream(tc.col_head + jc);
         ix.head = h;
         ix.cbase = ix.base + (int)(h.w & 0xffffu);
         ix.cnt = (int)((h.w >> 16) & 0xffu);
@@ -668,6 +684,9 @@ __device__ __forceinline__ RingIdx ring_load(const TileCtx& tc, const int2* __re
     }
     return ix;
 }
+
+__device__ __forceinline__ uint4 ldrec(const uint4* p) { return *p; }
+__device__ __forceinline__ uint2 ldrec(const uint2* p) { return ld_stream(p); }
 
 // Compact record fields (layout in mpk_plan.hpp).
 __device__ __forceinline__ uint32_t r8_elem(uint2 r) { return r.x & 0xffffffu; }
@@ -716,7 +735,7 @@ __device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned cha
         rec = tc.inc_ring + fbase + lane;
     Rec r[kB];
 #pragma unroll
-    for (int i = 0; i < kB; ++i) r[i] = rec[(i < klast ? i : klast) * kW];
+    for (int i = 0; i < kB; ++i) r[i] = ldrec(rec + (i < klast ? i : klast) * kW);
     tile_zero(op, acc, smem, tr);
     if (live) {
         Acc* col = acc + (cbase - base);
@@ -743,7 +762,7 @@ __device__ __forceinline__ void ring_run(const TileCtx& tc, Op& op, unsigned cha
 #pragma unroll
                     for (int i = 0; i < kB; ++i) {
                         const int k = k0 + i < cnt ? k0 + i : cnt - 1;
-                        r[i] = rec[k * kW];
+                        r[i] = ldrec(rec + k * kW);
                     }
                 }
                 Ids d[kB];
