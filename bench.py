@@ -82,7 +82,8 @@ def newton_workload(nx=100, ny=100, nz=67, seed=3):
 def cpu_baseline_newton(sample: int, reps: int):
     """Reference path on the host for the Newton set: the oracle's MEX-order
     Jacobian (J, R), curl-curl (K at the element nu) and mass kernels on
-    OpenMP threads + sparse() of each, on a bounded sample of the workload."""
+    OpenMP threads + sparse() of each (the oracle's threaded COO -> CSC, bit
+    for bit the serial one), on a bounded sample of the workload."""
     from oracle import oracle as O
     from magpackage_amd.fem_tables import get_quadrature_data, matlab_flat, nedelec_tet_p1
     from magpackage_amd.material import pack_material_data
@@ -103,19 +104,19 @@ def cpu_baseline_newton(sample: int, reps: int):
         I, J, V, Ri, Rv = O.jacobian(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], tags, a, qw,
                                      curl, packed)
         u = time.perf_counter()
-        Jm = O.sparse_scipy(I, J, V, n, n)
-        R = np.zeros(n)
-        np.add.at(R, Ri.astype(np.int64) - 1, Rv)
+        Jm = O.sparse_par(I, J, V, n, n)
+        keep = Ri > 0
+        R = np.bincount(Ri[keep].astype(np.int64) - 1, weights=Rv[keep], minlength=n)
         ts += time.perf_counter() - u
         B = O.element_b(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], a, curl)
         nu, _ = O.eval_nu((B * B).sum(axis=1), km.RegionTags, packed)
         trip = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl)
         u = time.perf_counter()
-        K = O.sparse_scipy(*trip, n, n)
+        K = O.sparse_par(*trip, n, n)
         ts += time.perf_counter() - u
         trip = O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sigma, qw, val)
         u = time.perf_counter()
-        M = O.sparse_scipy(*trip, n, n)
+        M = O.sparse_par(*trip, n, n)
         ts += time.perf_counter() - u
         del trip
         t1 = time.perf_counter()
@@ -134,7 +135,7 @@ def cpu_baseline_newton(sample: int, reps: int):
         "kind": "port",
         "sample": (f"Kuhn {sample}x{sample}x{nz} box ({km.num_elems} tets, same core and "
                    f"curve), Jacobian + B + curl-curl + mass kernels on {cores} OpenMP threads "
-                   f"+ serial sparse() of K, M, J, median of {reps} after 1 warm-up: {t:.3f} s "
+                   f"+ threaded sparse() of K, M, J, median of {reps} after 1 warm-up: {t:.3f} s "
                    f"(kernels {tk:.3f} s, sparse() {tsp:.3f} s)"),
     }
 
@@ -535,7 +536,8 @@ def calib_copy(dev):
 
 def cpu_baseline(sample_n: int, reps: int):
     """Reference path on the host: MEX-order COO kernels (oracle, OpenMP) +
-    sparse() for K and M_sigma + K + 1j*omega*M on a bounded sample."""
+    sparse() for K and M_sigma + K + 1j*omega*M on a bounded sample, the
+    sparse() and the sum on the oracle's threads as well."""
     from oracle import oracle as O
     from magpackage_amd.fem_tables import get_quadrature_data, matlab_flat, nedelec_tet_p1
     km, nu, sigma, omega = freq_workload(sample_n, seed=2)
@@ -549,14 +551,14 @@ def cpu_baseline(sample_n: int, reps: int):
         t0 = time.perf_counter()
         I, J, V = O.curl_curl(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], nu, qw, curl)
         t1 = time.perf_counter()
-        K = O.sparse_scipy(I, J, V, n, n)
+        K = O.sparse_par(I, J, V, n, n)
         del I, J, V
         t2 = time.perf_counter()
         I, J, V = O.mass(mi["P"], mi["T"], mi["Dofs"], mi["Signs"], sigma, qw, val)
         t3 = time.perf_counter()
-        M = O.sparse_scipy(I, J, V, n, n)
+        M = O.sparse_par(I, J, V, n, n)
         del I, J, V
-        A = K + 1j * omega * M
+        A = O.csc_add_jw(K, M, omega, n)
         t4 = time.perf_counter()
         return t4 - t0, (t1 - t0) + (t3 - t2), (t2 - t1) + (t4 - t3)
 
@@ -572,7 +574,7 @@ def cpu_baseline(sample_n: int, reps: int):
         "cores": cores,
         "kind": "port",
         "sample": (f"Kuhn {sample_n}^3 box ({km.num_elems} tets, same plate/sigma/50 Hz), "
-                   f"COO kernels on {cores} OpenMP threads + serial sparse() + K+1j*w*M, "
+                   f"COO kernels, sparse() and K+1j*w*M on {cores} OpenMP threads, "
                    f"median of {reps} after 1 warm-up: {t:.3f} s (COO kernels {tk:.3f} s, "
                    f"sparse() and K+1j*w*M {ts:.3f} s)"),
     }

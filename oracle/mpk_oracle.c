@@ -459,6 +459,145 @@ int64_t orc_sparse(const double* I, const double* J, const double* V, int64_t nt
     return nnz;
 }
 
+/*
+ * orc_sparse on OpenMP threads, for timing the host path at the core count
+ * of the machine: same contract and bit-identical output.  Triplets are
+ * counted per column and scattered into column buckets with atomics (in any
+ * order); each bucket is then sorted by (row, triplet index), which restores
+ * the serial order within every (column, row) group, so duplicates are
+ * summed in triplet order from +0.0 exactly as orc_sparse sums them.
+ */
+typedef struct { int64_t r, t; } orc_rt;
+static int orc_rt_cmp(const void* x, const void* y) {
+    const orc_rt* a = (const orc_rt*)x;
+    const orc_rt* b = (const orc_rt*)y;
+    if (a->r != b->r) return a->r < b->r ? -1 : 1;
+    return a->t < b->t ? -1 : (a->t > b->t ? 1 : 0);
+}
+static void orc_rt_sort(orc_rt* q, int64_t k) {
+    if (k > 48) {
+        qsort(q, (size_t)k, sizeof(orc_rt), orc_rt_cmp);
+        return;
+    }
+    for (int64_t i = 1; i < k; i++) {
+        const orc_rt v = q[i];
+        int64_t j = i - 1;
+        while (j >= 0 && orc_rt_cmp(&q[j], &v) > 0) { q[j + 1] = q[j]; j--; }
+        q[j + 1] = v;
+    }
+}
+
+int64_t orc_sparse_par(const double* I, const double* J, const double* V, int64_t nt,
+                       int64_t m, int64_t n, int drop_zeros, int64_t* colptr,
+                       int64_t* rowind, double* vals) {
+    int64_t* start = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    int64_t* ucnt = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    if (!start || !cur || !ucnt) { free(start); free(cur); free(ucnt); return -2; }
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t t = 0; t < nt; t++) {
+        if (I[t] > 0 && J[t] > 0) {
+            const int64_t r = (int64_t)I[t], c = (int64_t)J[t];
+            if (r > m || c > n) { bad = 1; continue; }
+#pragma omp atomic
+            start[c]++;
+        }
+    }
+    if (bad) { free(start); free(cur); free(ucnt); return -1; }
+    for (int64_t c = 0; c < n; c++) start[c + 1] += start[c];
+    const int64_t nv = start[n];
+    orc_rt* b = (orc_rt*)malloc(sizeof(orc_rt) * (size_t)(nv > 0 ? nv : 1));
+    double* sv = (double*)malloc(sizeof(double) * (size_t)(nv > 0 ? nv : 1));
+    if (!b || !sv) { free(start); free(cur); free(ucnt); free(b); free(sv); return -2; }
+    memcpy(cur, start, sizeof(int64_t) * (size_t)n);
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < nt; t++) {
+        if (I[t] > 0 && J[t] > 0) {
+            const int64_t c = (int64_t)J[t] - 1;
+            int64_t p;
+#pragma omp atomic capture
+            p = cur[c]++;
+            b[p].r = (int64_t)I[t] - 1;
+            b[p].t = t;
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t c = 0; c < n; c++) {
+        orc_rt* q = b + start[c];
+        const int64_t k = start[c + 1] - start[c];
+        orc_rt_sort(q, k);
+        int64_t u = 0;
+        for (int64_t i = 0; i < k;) {
+            const int64_t r = q[i].r;
+            double sum = 0.0;
+            while (i < k && q[i].r == r) { sum += V[q[i].t]; i++; }
+            if (!drop_zeros || sum != 0.0) {
+                q[u].r = r;  /* u <= the consumed index: overwrites a used slot */
+                sv[start[c] + u] = sum;
+                u++;
+            }
+        }
+        ucnt[c] = u;
+    }
+    colptr[0] = 0;
+    for (int64_t c = 0; c < n; c++) colptr[c + 1] = colptr[c] + ucnt[c];
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; c++) {
+        const orc_rt* q = b + start[c];
+        for (int64_t i = 0; i < ucnt[c]; i++) {
+            rowind[colptr[c] + i] = q[i].r;
+            vals[colptr[c] + i] = sv[start[c] + i];
+        }
+    }
+    const int64_t nnz = colptr[n];
+    free(start); free(cur); free(ucnt); free(b); free(sv);
+    return nnz;
+}
+
+/*
+ * K + 1j*w*M for two CSC matrices of the same n columns with sorted rows (the
+ * sum FrequencySolver.m forms after assembly): the union of the two patterns,
+ * re = K (0 where K has no entry), im = w * M (0 where M has none).  Two
+ * passes over the columns on OpenMP threads; returns the nnz of the sum.
+ */
+int64_t orc_csc_add_jw(int64_t n, const int64_t* kp, const int64_t* kr, const double* kv,
+                       const int64_t* mp, const int64_t* mr, const double* mv, double w,
+                       int64_t* colptr, int64_t* rowind, double* re, double* im) {
+    int64_t* ucnt = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    if (!ucnt) return -2;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; c++) {
+        int64_t a = kp[c], b = mp[c], u = 0;
+        while (a < kp[c + 1] || b < mp[c + 1]) {
+            if (b >= mp[c + 1] || (a < kp[c + 1] && kr[a] < mr[b])) a++;
+            else if (a >= kp[c + 1] || mr[b] < kr[a]) b++;
+            else { a++; b++; }
+            u++;
+        }
+        ucnt[c] = u;
+    }
+    colptr[0] = 0;
+    for (int64_t c = 0; c < n; c++) colptr[c + 1] = colptr[c] + ucnt[c];
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; c++) {
+        int64_t a = kp[c], b = mp[c], o = colptr[c];
+        while (a < kp[c + 1] || b < mp[c + 1]) {
+            if (b >= mp[c + 1] || (a < kp[c + 1] && kr[a] < mr[b])) {
+                rowind[o] = kr[a]; re[o] = kv[a]; im[o] = 0.0; a++;
+            } else if (a >= kp[c + 1] || mr[b] < kr[a]) {
+                rowind[o] = mr[b]; re[o] = 0.0; im[o] = w * mv[b]; b++;
+            } else {
+                rowind[o] = kr[a]; re[o] = kv[a]; im[o] = w * mv[b]; a++; b++;
+            }
+            o++;
+        }
+    }
+    const int64_t nnz = colptr[n];
+    free(ucnt);
+    return nnz;
+}
+
 /* ------------------------------------------------------------------ */
 /* BoundaryCondition.findOuterBoundaryDofs, Nedelec branch              */
 /* ------------------------------------------------------------------ */

@@ -49,6 +49,8 @@ def _i(a):
 _lib.orc_generate_edges.restype = _i64
 _lib.orc_distribute_dofs.restype = _i64
 _lib.orc_sparse.restype = _i64
+_lib.orc_sparse_par.restype = _i64
+_lib.orc_csc_add_jw.restype = _i64
 
 
 def set_threads(n: int) -> None:
@@ -235,6 +237,50 @@ def sparse(I, J, V, m, n, drop_zeros=True):
     if nnz < 0:
         raise IndexError("sparse(): index exceeds matrix dimensions")
     return colptr, rowind[:nnz].copy(), vals[:nnz].copy()
+
+
+def sparse_par(I, J, V, m, n, drop_zeros=True):
+    """sparse() on OpenMP threads (orc_sparse_par): the same (colptr, rowind0,
+    vals) as sparse(), bit for bit; used to time the host path."""
+    I, Ip = _d(I); J, Jp = _d(J); V, Vp = _d(V)
+    nt = I.size
+    colptr = np.zeros(n + 1, dtype=np.int64)
+    rowind = np.empty(max(nt, 1), dtype=np.int64)
+    vals = np.empty(max(nt, 1))
+    nnz = _lib.orc_sparse_par(Ip, Jp, Vp, _i64(nt), _i64(m), _i64(n),
+                              C.c_int(1 if drop_zeros else 0),
+                              colptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                              rowind.ctypes.data_as(C.POINTER(C.c_int64)), vals.ctypes.data_as(_dp))
+    if nnz == -1:
+        raise IndexError("sparse(): index exceeds matrix dimensions")
+    if nnz < 0:
+        raise MemoryError("sparse(): host allocation failed")
+    return colptr, rowind[:nnz], vals[:nnz]
+
+
+def csc_add_jw(k, mm, w, n):
+    """K + 1j*w*M of two CSC triples (colptr, rowind0, vals) with n columns:
+    returns (colptr, rowind0, complex vals) over the union of the patterns."""
+    kp, kr, kv = (np.ascontiguousarray(k[0], dtype=np.int64), np.ascontiguousarray(k[1], dtype=np.int64),
+                  np.ascontiguousarray(k[2], dtype=np.float64))
+    mp, mr, mv = (np.ascontiguousarray(mm[0], dtype=np.int64), np.ascontiguousarray(mm[1], dtype=np.int64),
+                  np.ascontiguousarray(mm[2], dtype=np.float64))
+    cap = max(kr.size + mr.size, 1)
+    colptr = np.zeros(n + 1, dtype=np.int64)
+    rowind = np.empty(cap, dtype=np.int64)
+    re = np.empty(cap)
+    im = np.empty(cap)
+    p64 = C.POINTER(C.c_int64)
+    nnz = _lib.orc_csc_add_jw(_i64(n), kp.ctypes.data_as(p64), kr.ctypes.data_as(p64),
+                              kv.ctypes.data_as(_dp), mp.ctypes.data_as(p64), mr.ctypes.data_as(p64),
+                              mv.ctypes.data_as(_dp), C.c_double(w), colptr.ctypes.data_as(p64),
+                              rowind.ctypes.data_as(p64), re.ctypes.data_as(_dp), im.ctypes.data_as(_dp))
+    if nnz < 0:
+        raise MemoryError("csc_add_jw: host allocation failed")
+    out = np.empty(nnz, dtype=np.complex128)
+    out.real = re[:nnz]
+    out.imag = im[:nnz]
+    return colptr, rowind[:nnz], out
 
 
 def sparse_scipy(I, J, V, m, n, drop_zeros=True):

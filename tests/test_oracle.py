@@ -301,6 +301,46 @@ def test_sparse_against_scipy():
         O.sparse(np.array([m + 1.0]), np.array([1.0]), np.array([1.0]), m, n)
 
 
+def test_sparse_par_matches_serial_bitwise():
+    """The threaded sparse() (used to time the host path) returns what the
+    serial restatement does, bit for bit: buckets past the insertion-sort
+    size, exact cancellations, filtered triplets, both drop modes."""
+    rng = np.random.default_rng(41)
+    for m, n, nt in ((40, 30, 6000), (5000, 4000, 20000), (3, 3, 0)):
+        I = rng.integers(0, m + 1, nt).astype(np.float64)
+        J = rng.integers(0, n + 1, nt).astype(np.float64)
+        V = rng.normal(size=nt)
+        if nt:
+            # exact cancellations: a value and its negative on the same entry
+            k = rng.choice(nt, size=nt // 10, replace=False)
+            I = np.concatenate([I, I[k]]); J = np.concatenate([J, J[k]]); V = np.concatenate([V, -V[k]])
+        for dz in (True, False):
+            c1, r1, v1 = O.sparse(I, J, V, m, n, drop_zeros=dz)
+            c2, r2, v2 = O.sparse_par(I, J, V, m, n, drop_zeros=dz)
+            assert np.array_equal(c1, c2) and np.array_equal(r1, r2)
+            assert np.array_equal(v1.view(np.int64), v2.view(np.int64))
+    with pytest.raises(IndexError):
+        O.sparse_par(np.array([1.0]), np.array([n + 1.0]), np.array([1.0]), m, n)
+
+
+def test_csc_add_jw_against_scipy():
+    rng = np.random.default_rng(42)
+    m = n = 60
+    w = 2 * np.pi * 50.0
+    mats = []
+    for nt in (900, 150):
+        I = rng.integers(1, m + 1, nt).astype(np.float64)
+        J = rng.integers(1, n + 1, nt).astype(np.float64)
+        mats.append(O.sparse(I, J, rng.normal(size=nt), m, n))
+    c, r, v = O.csc_add_jw(mats[0], mats[1], w, n)
+    K = sp.csc_matrix((mats[0][2], mats[0][1], mats[0][0]), shape=(m, n))
+    M = sp.csc_matrix((mats[1][2], mats[1][1], mats[1][0]), shape=(m, n))
+    ref = (K + 1j * w * M).tocsc()
+    ref.sort_indices()
+    assert np.array_equal(c, ref.indptr) and np.array_equal(r, ref.indices)
+    assert np.array_equal(v, ref.data)
+
+
 # ------------------------------------------------------ topology / DOFs -----
 def test_edges_against_numpy_unique():
     mesh = kuhn_box(5, 4, 3, seed=19)
