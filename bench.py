@@ -634,7 +634,9 @@ def main():
                     help="newton4m: K, J, R in one pass and M_sigma in a second one")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp64_exact", "fp32_local"],
                     help="default fp64 (freq128) / fp32_local (newton4m)")
-    ap.add_argument("--cpu-sample", type=int, default=80)
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="cube count of the CPU baseline's box (default: the workload's own "
+                         "size, 128 for freq128 and 100 for newton4m)")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true",
@@ -659,10 +661,12 @@ def main():
         args.precision = args.precision or "fp32_local"
         if args.precision == "fp64_exact":
             ap.error("the fused Newton set has fp64 and fp32_local modes")
-        if args.cpu_sample == 80:
-            args.cpu_sample = 64
+        if args.cpu_sample is None:
+            args.cpu_sample = 100
         return run_newton(args)
     args.precision = args.precision or "fp64"
+    if args.cpu_sample is None:
+        args.cpu_sample = args.n
 
     import torch
     from magpackage_amd import _lib as L
@@ -759,7 +763,7 @@ def main():
         # with its FP64 time and error), roofline, traffic and CPU baseline
         na = argparse.Namespace(**vars(args))
         na.precision, na.calib = "fp32_local", False
-        na.cpu_sample = 64
+        na.cpu_sample = 100
         newton = run_newton(na, emit=False)
         for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline"):
             newton.pop(k, None)
