@@ -465,7 +465,9 @@ void launch_list_k(int kb, unsigned grid, size_t lds, hipStream_t st, const Tile
 // (list[0, *count)): as many blocks as fit on the device at the op's LDS size.
 template <class Op>
 int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t st,
-                     const int32_t* list, const int32_t* count, size_t max_per_cu = 16) {
+                     const int32_t* list, const int32_t* count, size_t max_per_cu = 16,
+                     int kb = 0) {
+    if (kb == 0) kb = ring_batch<Op>();
     size_t lds = 0;
     if (int rc = lds_bytes(plan, op, acc_size, 0, &lds)) return rc;
     if (plan->num_cus == 0) {
@@ -486,11 +488,9 @@ int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t s
     const unsigned grid = (unsigned)(g > 8 ? g : 8);
     const TileCtx tc = tile_ctx(plan);
     if (plan->ring8)
-        launch_list_k<Op, true>(ring_batch<Op>(), grid, lds, st, tc, plan->col_ends, op, list,
-                                count);
+        launch_list_k<Op, true>(kb, grid, lds, st, tc, plan->col_ends, op, list, count);
     else
-        launch_list_k<Op, false>(ring_batch<Op>(), grid, lds, st, tc, plan->col_ends, op, list,
-                                 count);
+        launch_list_k<Op, false>(kb, grid, lds, st, tc, plan->col_ends, op, list, count);
     MPK_HIP_TRY(hipGetLastError());
     return MPK_OK;
 }
@@ -611,8 +611,16 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     // and a mass pass measured 0.705-0.708 against 0.699 ms).  FP32 state:
     // the two-plane pass and the mass pass, 4 blocks per CU (0.657 ms; the
     // same two passes after the class-0 pass 0.663-0.664 ms, the three-plane
-    // pass after it 0.663-0.665 ms).  MPK_NEWTON_OVERLAP=0/1,
-    // MPK_NEWTON_SPLIT_M=0/1 and MPK_LIST_BLOCKS_PER_CU (1..16) override.
+    // pass after it 0.663-0.665 ms).  The side passes get SIMD slots while
+    // the one-plane pass runs only where their waves' registers fit beside
+    // the one-plane waves: FP32, one side wave beside two one-plane waves
+    // (192 + 2 x 160 of 512 VGPRs); FP64, beside one.  A build with 168 for
+    // the one-plane pass starved them until its grid drained (0.72 ms);
+    // tests/test_kernel_resources.py checks these budgets on the built
+    // library.  MPK_NEWTON_OVERLAP=0/1, MPK_NEWTON_SPLIT_M=0/1,
+    // MPK_LIST_BLOCKS_PER_CU (1..16) and MPK_LIST_RING_BATCH (incidences per
+    // gather round of the side passes: 2, 3, 4, 6, 8; 4 and 3 measured
+    // within 0.4% of the default) override.
     bool overlap = true;
     if (const char* e = std::getenv("MPK_NEWTON_OVERLAP")) overlap = std::atoi(e) != 0;
     bool split_m = kPrec == MPK_PREC_FP32_LOCAL;
@@ -621,6 +629,11 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
     if (const char* e = std::getenv("MPK_LIST_BLOCKS_PER_CU")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 16) per_cu = (size_t)v;
+    }
+    int side_kb = 0;
+    if (const char* e = std::getenv("MPK_LIST_RING_BATCH")) {
+        const int v = std::atoi(e);
+        if (v == 2 || v == 3 || v == 4 || v == 6 || v == 8) side_kb = v;
     }
     if (noalloc) {
         overlap = false;
@@ -653,16 +666,16 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
                 return launch_ring_list(plan,
                                         FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals, j_vals,
                                                                      residual, qk, qm, 0, 0},
-                                        3 * sizeof(double), s, list, count, per_cu);
+                                        3 * sizeof(double), s, list, count, per_cu, side_kb);
             if (int rc = launch_ring_list(plan,
                                           FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr,
                                                                         j_vals, residual, qk, qm,
                                                                         0, 0},
-                                          2 * sizeof(double), s, list, count, per_cu))
+                                          2 * sizeof(double), s, list, count, per_cu, side_kb))
                 return rc;
             if (!with_m) return (int)MPK_OK;
             return launch_ring_list(plan, FrameMass<kPrec>{sigma, m_vals, qm, 0}, sizeof(double),
-                                    s, list, count, per_cu);
+                                    s, list, count, per_cu, side_kb);
         };
         auto class0 = [&]() {
             FrameNewtonLin<kPrec, SV> op{sv, k_vals, mfill ? nullptr : m_vals, j_vals, residual,
