@@ -495,6 +495,52 @@ int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t s
     return MPK_OK;
 }
 
+// The two-op persistent list launch (k_csc_ring_list2): op1 then op2 on each
+// listed tile, each with its own gather batch; per-CU block count from the
+// larger LDS reservation.
+template <class Op1, class Op2>
+int launch_ring_list2(const mpk_plan* plan, Op1 op1, size_t acc1, Op2 op2, size_t acc2,
+                      hipStream_t st, const int32_t* list, const int32_t* count,
+                      size_t max_per_cu, int kb1) {
+    size_t l1 = 0, l2 = 0;
+    if (int rc = lds_bytes(plan, op1, acc1, 0, &l1)) return rc;
+    if (int rc = lds_bytes(plan, op2, acc2, 0, &l2)) return rc;
+    const size_t lds = l1 > l2 ? l1 : l2;
+    if (plan->num_cus == 0) {
+        int cus = 0;
+        MPK_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                          plan->device));
+        plan->num_cus = cus > 0 ? cus : 256;
+    }
+    constexpr size_t kLdsPerCu = 160 * 1024;
+    size_t per_cu = lds ? kLdsPerCu / lds : 16;
+    per_cu = per_cu < 1 ? 1 : (per_cu > max_per_cu ? max_per_cu : per_cu);
+    const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
+    int64_t g = (int64_t)plan->num_cus * (int64_t)per_cu;
+    g = g < ntiles ? g : ntiles;
+    g = (g + 7) / 8 * 8;
+    const unsigned grid = (unsigned)(g > 8 ? g : 8);
+    const TileCtx tc = tile_ctx(plan);
+    constexpr int kB2 = RingDefault<Op2>::value;
+    if (kb1 == 6) {
+        if (plan->ring8)
+            hipLaunchKernelGGL((k_csc_ring_list2<Op1, Op2, 6, kB2, true>), dim3(grid), dim3(kW),
+                               lds, st, tc, plan->col_ends, op1, op2, list, count);
+        else
+            hipLaunchKernelGGL((k_csc_ring_list2<Op1, Op2, 6, kB2, false>), dim3(grid), dim3(kW),
+                               lds, st, tc, plan->col_ends, op1, op2, list, count);
+    } else {
+        if (plan->ring8)
+            hipLaunchKernelGGL((k_csc_ring_list2<Op1, Op2, 4, kB2, true>), dim3(grid), dim3(kW),
+                               lds, st, tc, plan->col_ends, op1, op2, list, count);
+        else
+            hipLaunchKernelGGL((k_csc_ring_list2<Op1, Op2, 4, kB2, false>), dim3(grid), dim3(kW),
+                               lds, st, tc, plan->col_ends, op1, op2, list, count);
+    }
+    MPK_HIP_TRY(hipGetLastError());
+    return MPK_OK;
+}
+
 // sel / want: ring driver only, run the tiles whose class flag equals want.
 template <class Op>
 int launch_frame(const mpk_plan* plan, const double* P, Op op, size_t acc_size, size_t extra,
@@ -630,6 +676,14 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
         const int v = std::atoi(e);
         if (v >= 1 && v <= 16) per_cu = (size_t)v;
     }
+    // FP32 listed tiles: the two-plane op and the mass op one after the other
+    // on each tile in one persistent launch (k_csc_ring_list2, the mass op's
+    // records from L2: 12 MB fewer fetched, the same 0.654-0.658 ms), with 4
+    // incidences per gather round for the two-plane op so that a wave (169
+    // VGPRs) fits beside two one-plane waves; with 6 (197 VGPRs) it did not
+    // and the step took 0.706-0.709 ms.  MPK_NEWTON_LIST2=0: two launches.
+    bool list2 = true;
+    if (const char* e = std::getenv("MPK_NEWTON_LIST2")) list2 = std::atoi(e) != 0;
     int side_kb = 0;
     if (const char* e = std::getenv("MPK_LIST_RING_BATCH")) {
         const int v = std::atoi(e);
@@ -667,6 +721,13 @@ int newton_set(const mpk_plan* plan, const double* P, const int32_t* tags, const
                                         FrameNewton<kPrec, true, SV>{sv, k_vals, m_vals, j_vals,
                                                                      residual, qk, qm, 0, 0},
                                         3 * sizeof(double), s, list, count, per_cu, side_kb);
+            if (with_m && list2)
+                return launch_ring_list2(plan,
+                                         FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr, j_vals,
+                                                                       residual, qk, qm, 0, 0},
+                                         2 * sizeof(double),
+                                         FrameMass<kPrec>{sigma, m_vals, qm, 0}, sizeof(double),
+                                         s, list, count, per_cu, side_kb == 6 ? 6 : 4);
             if (int rc = launch_ring_list(plan,
                                           FrameNewton<kPrec, false, SV>{sv, k_vals, nullptr,
                                                                         j_vals, residual, qk, qm,

@@ -863,6 +863,30 @@ __global__ void __launch_bounds__(kW) k_csc_ring_list(TileCtx tc, const int2* __
     }
 }
 
+// Two ops over the tiles of a device list, one after the other on each tile
+// (the second op's records and gathers come back from L2): the Newton set's
+// listed tiles, K, J and R from the two-plane op and M from the mass op, in
+// one persistent launch.  The LDS reservation is the larger of the two ops'.
+template <class Op1, class Op2, int kB1, int kB2, bool kC>
+__global__ void __launch_bounds__(kW) k_csc_ring_list2(TileCtx tc, const int2* __restrict__ ends,
+                                                       Op1 op1, Op2 op2,
+                                                       const int32_t* __restrict__ list,
+                                                       const int32_t* __restrict__ count) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int64_t kXcd = 8;
+    const int64_t c = *count;
+    const int64_t x = blockIdx.x % kXcd;
+    const int64_t nbx = ((int64_t)gridDim.x - x + kXcd - 1) / kXcd;  // blocks on XCD x
+    const int64_t lo = c * x / kXcd, hi = c * (x + 1) / kXcd;
+    for (int64_t p = lo + blockIdx.x / kXcd; p < hi; p += nbx) {
+        const int64_t t = list[p];
+        ring_tile<Op1, kB1, kC>(tc, ends, op1, smem, t);
+        __syncthreads();
+        ring_tile<Op2, kB2, kC>(tc, ends, op2, smem, t);
+        __syncthreads();
+    }
+}
+
 // K + j w_f M_sigma for several frequencies from one pass: the LDS tile holds
 // (K, M) as FrameKM's does; the store writes one complex128 value set per
 // frequency, laid out [f][nnz] (HBFEM-style harmonic blocks, frequency sweeps).

@@ -2,9 +2,10 @@
 from the built library's gfx950 code objects (AMDGPU metadata, .vgpr_count).
 
 A SIMD holds 512 VGPRs per lane, allocated in granules of 8.
-- FP32-local Newton set: each listed-tile pass on the side stream runs beside
-  the one-plane pass only where a SIMD has room for one of its waves next to
-  two one-plane waves. A build in which the one-plane pass took 168 VGPRs
+- FP32-local Newton set: the listed-tile pass on the side stream (one fused
+  launch by default, two launches as the alternative) runs beside the
+  one-plane pass only where a SIMD has room for one of its waves next to two
+  one-plane waves. A build in which the one-plane pass took 168 VGPRs
   instead of 158 (same occupancy) starved the side passes until the one-plane
   grid drained: 0.655 -> 0.72 ms on the 4M-tet set.
 - FP64 Newton set: the three-plane side pass fits beside one one-plane wave.
@@ -76,6 +77,14 @@ def test_fp32_newton_side_passes_fit_beside_two_one_plane_waves(vgprs):
     mass = _one(vgprs, "k_csc_ring_list", "FrameMassILi2EEELi3ELb1E")
     assert 2 * _alloc(lin) + _alloc(kd) <= 512, (lin, kd)
     assert 2 * _alloc(lin) + _alloc(mass) <= 512, (lin, mass)
+
+
+def test_fp32_newton_fused_side_pass_fits_beside_two_one_plane_waves(vgprs):
+    # the default listed-tile launch: the two-plane op (4 incidences per
+    # gather round) and the mass op in one kernel
+    lin = _one(vgprs, "k_csc_ring_batch", "FrameNewtonLinILi2E", "Li6ELb1E")
+    fused = _one(vgprs, "k_csc_ring_list2", "FrameNewtonILi2ELb0E", "Li4ELi3ELb1E")
+    assert 2 * _alloc(lin) + _alloc(fused) <= 512, (lin, fused)
 
 
 def test_fp64_newton_side_pass_fits_beside_one_one_plane_wave(vgprs):
