@@ -461,15 +461,9 @@ void launch_list_k(int kb, unsigned grid, size_t lds, hipStream_t st, const Tile
     }
 }
 
-// Persistent launch of the ring driver over the tiles of a device list
-// (list[0, *count)): as many blocks as fit on the device at the op's LDS size.
-template <class Op>
-int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t st,
-                     const int32_t* list, const int32_t* count, size_t max_per_cu = 16,
-                     int kb = 0) {
-    if (kb == 0) kb = ring_batch<Op>();
-    size_t lds = 0;
-    if (int rc = lds_bytes(plan, op, acc_size, 0, &lds)) return rc;
+// Grid of a persistent list launch: as many blocks as fit on the device at
+// `lds` bytes each (at most max_per_cu per CU), no more than the plan's tiles.
+int list_grid(const mpk_plan* plan, size_t lds, size_t max_per_cu, unsigned* grid) {
     if (plan->num_cus == 0) {
         int cus = 0;
         MPK_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
@@ -480,12 +474,26 @@ int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t s
     size_t per_cu = lds ? kLdsPerCu / lds : 16;
     per_cu = per_cu < 1 ? 1 : (per_cu > max_per_cu ? max_per_cu : per_cu);
     const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
-    // the kernel splits the list over the 8 XCDs by block index: the grid is a
-    // multiple of 8 so that every XCD's range has blocks
+    // the kernels split the list over the 8 XCDs by block index: the grid is
+    // a multiple of 8 so that every XCD's range has blocks
     int64_t g = (int64_t)plan->num_cus * (int64_t)per_cu;
     g = g < ntiles ? g : ntiles;
     g = (g + 7) / 8 * 8;
-    const unsigned grid = (unsigned)(g > 8 ? g : 8);
+    *grid = (unsigned)(g > 8 ? g : 8);
+    return MPK_OK;
+}
+
+// Persistent launch of the ring driver over the tiles of a device list
+// (list[0, *count)): as many blocks as fit on the device at the op's LDS size.
+template <class Op>
+int launch_ring_list(const mpk_plan* plan, Op op, size_t acc_size, hipStream_t st,
+                     const int32_t* list, const int32_t* count, size_t max_per_cu = 16,
+                     int kb = 0) {
+    if (kb == 0) kb = ring_batch<Op>();
+    size_t lds = 0;
+    if (int rc = lds_bytes(plan, op, acc_size, 0, &lds)) return rc;
+    unsigned grid = 0;
+    if (int rc = list_grid(plan, lds, max_per_cu, &grid)) return rc;
     const TileCtx tc = tile_ctx(plan);
     if (plan->ring8)
         launch_list_k<Op, true>(kb, grid, lds, st, tc, plan->col_ends, op, list, count);
@@ -506,20 +514,8 @@ int launch_ring_list2(const mpk_plan* plan, Op1 op1, size_t acc1, Op2 op2, size_
     if (int rc = lds_bytes(plan, op1, acc1, 0, &l1)) return rc;
     if (int rc = lds_bytes(plan, op2, acc2, 0, &l2)) return rc;
     const size_t lds = l1 > l2 ? l1 : l2;
-    if (plan->num_cus == 0) {
-        int cus = 0;
-        MPK_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
-                                          plan->device));
-        plan->num_cus = cus > 0 ? cus : 256;
-    }
-    constexpr size_t kLdsPerCu = 160 * 1024;
-    size_t per_cu = lds ? kLdsPerCu / lds : 16;
-    per_cu = per_cu < 1 ? 1 : (per_cu > max_per_cu ? max_per_cu : per_cu);
-    const int64_t ntiles = (plan->n + kTileCols - 1) / kTileCols;
-    int64_t g = (int64_t)plan->num_cus * (int64_t)per_cu;
-    g = g < ntiles ? g : ntiles;
-    g = (g + 7) / 8 * 8;
-    const unsigned grid = (unsigned)(g > 8 ? g : 8);
+    unsigned grid = 0;
+    if (int rc = list_grid(plan, lds, max_per_cu, &grid)) return rc;
     const TileCtx tc = tile_ctx(plan);
     constexpr int kB2 = RingDefault<Op2>::value;
     if (kb1 == 6) {
